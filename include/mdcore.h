@@ -1,0 +1,186 @@
+/*
+ * mdcore.h -- C ABI of the MI355X (gfx950) molecular-dynamics core.
+ *
+ * This is the drop-in boundary for the upstream project's WebGPU compute path
+ * (pablo-arantes/making-it-rain-webgpu, `src/gpu/`).  The host keeps the
+ * `src/engine` API (`ProtocolEngine`, `src/engine/protocol.ts:32-62`); each
+ * entry point below names the upstream interface it replaces.  Plain C types
+ * only: flat arrays in, flat arrays out, no retained caller pointers (every
+ * array is copied on the call), opaque handles own all device memory.
+ *
+ * Units are OpenMM's: nm, ps, amu, kJ/mol, e, K, bar.  Positions, velocities
+ * and forces are xyz-interleaved double[3N].  A box is nine doubles, the row
+ * vectors a, b, c; it is reduced to OpenMM's lower-triangular form on entry.
+ *
+ * Threading: calls on one handle must come from one host thread at a time.
+ * Every call is blocking; work is issued on the handle's own HIP stream.
+ */
+#ifndef MDCORE_H
+#define MDCORE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes.  The named JavaScript errors they map to upstream:
+ * UnsupportedForce (src/engine/system.ts:178), CellListTooCoarse
+ * (src/gpu/nonbonded.ts:54-65), BoxTooSmallForCells (src/gpu/cell-list.ts:41-64),
+ * ClusterTooLarge (src/engine/constraint-clusters.ts:71-83), NotSmooth
+ * (src/reference/fft.ts:24), CutoffTooLargeForBox (src/reference/triclinic.ts:179),
+ * the fixed-point RangeError (src/reference/fixed-point.ts:56-66). */
+typedef enum {
+  MD_OK = 0,
+  MD_ERR_UNSUPPORTED_FORCE = 1,
+  MD_ERR_CUTOFF_TOO_LARGE = 2,
+  MD_ERR_CLUSTER_TOO_LARGE = 3,
+  MD_ERR_FIXED_POINT_OVERFLOW = 4,
+  MD_ERR_HIP = 5,
+  MD_ERR_OUT_OF_MEMORY = 6,
+  MD_ERR_INVALID_ARGUMENT = 7,
+  MD_ERR_LIST_OVERFLOW = 8,
+  MD_ERR_NOT_SMOOTH = 9,
+  MD_ERR_CONSTRAINTS_DIVERGED = 10,
+  MD_ERR_FAULT_INJECTED = 11,
+  MD_ERR_NO_DEVICE = 12
+} md_err;
+
+/* Per-force-group energies returned by md_evaluate, kJ/mol.  Indices follow
+ * the group names of src/reference/potential.ts:101-142. */
+enum {
+  MD_GROUP_BOND = 0,       /* HarmonicBondForce */
+  MD_GROUP_ANGLE = 1,      /* HarmonicAngleForce */
+  MD_GROUP_TORSION = 2,    /* PeriodicTorsionForce */
+  MD_GROUP_DIRECT = 3,     /* NonbondedForce[direct]: pairs + exclusion
+                              correction + exceptions + dispersion */
+  MD_GROUP_RECIPROCAL = 4, /* NonbondedForce[reciprocal]: mesh + self + net charge */
+  MD_GROUP_RESTRAINT = 5,  /* PositionRestraint */
+  MD_GROUP_COUNT = 6
+};
+
+/* An OpenMM System, flattened (src/engine/system.ts:20-173).  Counts of zero
+ * leave the matching arrays unread.  All arrays are copied. */
+typedef struct {
+  int32_t n_particles;
+  const double *masses; /* amu; 0 = frozen */
+  double box[9];        /* default periodic box vectors */
+
+  int32_t n_constraints;
+  const int32_t *constraint_p1, *constraint_p2;
+  const double *constraint_distance;
+
+  int32_t n_bonds; /* HarmonicBondForce: E = k/2 (r - d)^2 */
+  const int32_t *bond_p1, *bond_p2;
+  const double *bond_length, *bond_k;
+
+  int32_t n_angles; /* HarmonicAngleForce: E = k/2 (theta - a)^2 */
+  const int32_t *angle_p1, *angle_p2, *angle_p3;
+  const double *angle_theta, *angle_k;
+
+  int32_t n_torsions; /* PeriodicTorsionForce: E = k (1 + cos(n phi - phase)) */
+  const int32_t *torsion_p1, *torsion_p2, *torsion_p3, *torsion_p4, *torsion_periodicity;
+  const double *torsion_phase, *torsion_k;
+
+  int32_t has_nonbonded; /* NonbondedForce with method PME */
+  double cutoff, ewald_tolerance, alpha; /* alpha 0: derived from cutoff and tolerance */
+  int32_t grid[3];                       /* 0: derived from the box diagonal */
+  int32_t use_dispersion_correction;
+  const double *charge, *sigma, *epsilon;
+  int32_t n_exceptions;
+  const int32_t *exception_p1, *exception_p2;
+  const double *exception_chargeprod, *exception_sigma, *exception_epsilon;
+
+  int32_t n_restraints; /* CustomExternalForce k*periodicdistance(x,y,z,x0,y0,z0)^2 */
+  const int32_t *restraint_particle;
+  const double *restraint_k, *restraint_x0, *restraint_y0, *restraint_z0;
+
+  int32_t cmm_frequency; /* CMMotionRemover frequency; 0 when absent */
+
+  int32_t has_barostat; /* MonteCarloBarostat */
+  double barostat_pressure, barostat_temperature;
+  int32_t barostat_frequency;
+  uint32_t barostat_seed;
+} md_system_desc;
+
+/* Run options (RunOptions, src/gpu/loop.ts:117-168). */
+typedef struct {
+  double time_step;            /* ps */
+  double temperature;          /* K */
+  double friction;             /* 1/ps */
+  double constraint_tolerance; /* relative */
+  uint32_t seed;
+  int32_t integrator;          /* 0 LangevinMiddle, 1 Verlet (NVE gate) */
+  double verlet_skin;          /* nm; < 0 picks the default (0.15, nonbonded.ts:227) */
+  int32_t mesh[3];             /* PME mesh override; 0 = from the system or the box */
+  int32_t device;              /* HIP device ordinal */
+} md_run_options;
+
+typedef struct md_system md_system;
+
+/* Last error message on this thread ("" when none). */
+const char *md_last_error(void);
+/* Library version string. */
+const char *md_version(void);
+
+/* StepLoop.fromParsed (src/gpu/loop.ts:278-532): validate, pack and upload a
+ * System; positions and velocities start at zero until set. */
+md_err md_system_create(const md_system_desc *desc, const md_run_options *options,
+                        md_system **out);
+/* StepLoop.destroy (src/gpu/loop.ts:1113-1115). */
+void md_system_destroy(md_system *sys);
+
+/* StepLoop.setPositions / setVelocities (src/gpu/loop.ts:819-855).
+ * setPositions clears the integrator's carried residual, as upstream. */
+md_err md_set_positions(md_system *sys, const double *positions);
+md_err md_set_velocities(md_system *sys, const double *velocities);
+/* StepLoop.positions / velocities (src/gpu/loop.ts:900-915). */
+md_err md_get_positions(md_system *sys, double *positions);
+md_err md_get_velocities(md_system *sys, double *velocities);
+/* StepLoop.setBox (src/gpu/loop.ts:1031-1097): every box-dependent constant
+ * is recomputed; the mesh size stays fixed, as upstream. */
+md_err md_set_box(md_system *sys, const double box[9]);
+md_err md_get_box(md_system *sys, double box[9]);
+
+/* StepLoop.potential / evaluateForces / report (src/gpu/loop.ts:761-930):
+ * forces (kJ/mol/nm, may be NULL) and per-group energies (MD_GROUP_COUNT,
+ * may be NULL) at the current positions; returns the total in *total. */
+md_err md_evaluate(md_system *sys, double *forces, double *groups, double *total);
+
+/* StepLoop.run (src/gpu/loop.ts:651-685): advance `steps` steps.  Strides are
+ * captured once as HIP graphs and replayed; the thermostat's step counter
+ * lives on the device. */
+md_err md_run(md_system *sys, int64_t steps);
+
+/* Steps taken so far, and StepLoop.advanceStepCount (src/gpu/loop.ts:554-569). */
+md_err md_get_step(md_system *sys, int64_t *step);
+md_err md_set_step(md_system *sys, int64_t step);
+
+/* StepLoop.verletFlags (src/gpu/loop.ts:941-960). */
+typedef struct {
+  int64_t rebuilds;
+  int64_t tiles;            /* cluster-pair tiles in the current list */
+  int64_t tile_capacity;
+  double worst_displacement;/* nm, since the last build */
+  int32_t overflowed;
+} md_list_flags;
+md_err md_list_flags_get(md_system *sys, md_list_flags *flags);
+
+/* Work counts the benchmark uses: in-cutoff pair count (exact, counted on the
+ * device at the current positions) and cluster tiles computed. */
+md_err md_pair_census(md_system *sys, int64_t *pairs_within_cutoff, int64_t *tiles);
+
+/* Diagnostic probes (the upstream `random-probe` kernel, K41, and its
+ * fixed-point twin): run the device primitives on host-given inputs. */
+md_err md_probe_fixed_point(const float *values, int32_t n, int64_t *fixed, float *back);
+md_err md_probe_thermostat(uint32_t seed, uint32_t step, int32_t n, uint32_t *bits, float *normals);
+
+/* Fault injection for host-side failure tests: the next `after` kernel
+ * launches succeed, then every call returns MD_ERR_FAULT_INJECTED until
+ * re-armed with a negative value.  No kernel is altered. */
+md_err md_inject_fault(md_system *sys, int32_t after);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,821 @@
+// api.hip -- the C ABI of include/mdcore.h: System validation and packing,
+// uploads and readbacks, force evaluation and the step loop.
+//
+// Host-side counterpart of StepLoop (src/gpu/loop.ts:208-1116) and the
+// per-force `prepare*` modules (nonbonded.ts, pme.ts, constrain.ts,
+// forces.ts, restraint.ts, integrate.ts).  A step is a fixed sequence of
+// kernel launches on the system's stream; a stride of steps is captured once
+// as a HIP graph and replayed, with the step counter, rebuild decision and
+// COM momentum held in device memory so replays need no host round trip.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+
+static thread_local std::string g_last_error;
+
+static md_err fail(const md_error &e) {
+  g_last_error = e.message;
+  return e.code;
+}
+
+#define MD_API_BEGIN try {
+#define MD_API_END                                       \
+  }                                                      \
+  catch (const md_error &e) {                            \
+    return fail(e);                                      \
+  }                                                      \
+  catch (const std::bad_alloc &) {                       \
+    g_last_error = "host allocation failed";             \
+    return MD_ERR_OUT_OF_MEMORY;                         \
+  }                                                      \
+  catch (const std::exception &e) {                      \
+    g_last_error = e.what();                             \
+    return MD_ERR_INVALID_ARGUMENT;                      \
+  }                                                      \
+  g_last_error.clear();                                  \
+  return MD_OK;
+
+static const int STRIDE_STEPS = 50;
+
+// ---- cell geometry (src/reference/triclinic.ts) ------------------------------
+
+// ties to even, as reduceBox (triclinic.ts:93-125)
+static double reduce_by(double ratio) {
+  double nearest = std::floor(ratio + 0.5);
+  if (std::fabs(ratio - std::trunc(ratio)) == 0.5 && std::fmod(nearest, 2.0) != 0.0)
+    return nearest - (ratio > 0 ? 1.0 : -1.0);
+  return nearest;
+}
+
+static void reduce_box(const double *in, double *out) {
+  double v[9];
+  std::memcpy(v, in, sizeof v);
+  for (int k : {1, 2, 5})
+    if (std::fabs(v[k]) > 1e-9)
+      throw md_error(MD_ERR_INVALID_ARGUMENT, "box vectors must be lower triangular: a along x, b in the xy plane");
+  if (v[0] <= 0 || v[4] <= 0 || v[8] <= 0)
+    throw md_error(MD_ERR_INVALID_ARGUMENT, "box vectors need positive diagonal elements");
+  double t = reduce_by(v[7] / v[4]);
+  for (int k = 0; k < 3; ++k) v[6 + k] -= t * v[3 + k];
+  t = reduce_by(v[6] / v[0]);
+  for (int k = 0; k < 3; ++k) v[6 + k] -= t * v[k];
+  t = reduce_by(v[3] / v[0]);
+  for (int k = 0; k < 3; ++k) v[3 + k] -= t * v[k];
+  v[1] = v[2] = v[5] = 0.0;
+  std::memcpy(out, v, sizeof v);
+}
+
+static double box_volume(const double *b) { return b[0] * b[4] * b[8]; }
+
+static double min_perpendicular_width(const double *b) {
+  double v = box_volume(b);
+  auto cn = [](const double *u, const double *w) {
+    double x = u[1] * w[2] - u[2] * w[1], y = u[2] * w[0] - u[0] * w[2], z = u[0] * w[1] - u[1] * w[0];
+    return std::sqrt(x * x + y * y + z * z);
+  };
+  return std::min(v / cn(b + 3, b + 6), std::min(v / cn(b + 6, b), v / cn(b, b + 3)));
+}
+
+static int64_t fixed_checked(double x, const char *what) {
+  if (!std::isfinite(x) || std::fabs(x) >= 2147483648.0)
+    throw md_error(MD_ERR_FIXED_POINT_OVERFLOW,
+                   std::string(what) + " " + std::to_string(x) +
+                       " is outside the fixed-point range of +/-2^31; a value this large is almost always a "
+                       "diverged simulation rather than a format limit");
+  return md_double_to_fixed(x);
+}
+
+static void apply_box(md_system *s, const double *in) {
+  double b[9];
+  reduce_box(in, b);
+  if (s->has_nonbonded) {
+    double maxc = 0.5 * min_perpendicular_width(b);
+    if (s->cutoff > maxc)
+      throw md_error(MD_ERR_CUTOFF_TOO_LARGE,
+                     "a " + std::to_string(s->cutoff) + " nm cutoff does not fit this box: the minimum image is only "
+                     "exact below " + std::to_string(maxc) + " nm (half the narrowest perpendicular width)");
+  }
+  std::memcpy(s->box, b, sizeof b);
+  md_box &d = s->dbox;
+  std::memset(&d, 0, sizeof d);
+  for (int k = 0; k < 3; ++k) {
+    d.fa[k] = fixed_checked(b[k], "box");
+    d.fb[k] = fixed_checked(b[3 + k], "box");
+    d.fc[k] = fixed_checked(b[6 + k], "box");
+    d.a[k] = (float)b[k];
+    d.b[k] = (float)b[3 + k];
+    d.c[k] = (float)b[6 + k];
+  }
+  double ax = b[0], bx = b[3], by = b[4], cx = b[6], cy = b[7], cz = b[8];
+  double m[9] = {1.0 / ax, 0, 0, -bx / (ax * by), 1.0 / by, 0, ((bx * cy) / (by * cz) - cx / cz) / ax,
+                 -cy / (by * cz), 1.0 / cz};
+  for (int k = 0; k < 9; ++k) {
+    d.invd[k] = m[k];
+    d.inv[k] = (float)m[k];
+  }
+  d.rectangular = bx == 0 && cx == 0 && cy == 0;
+  // any captured stride bakes the box into its kernel arguments
+  if (s->stride_exec) {
+    hipGraphExecDestroy(s->stride_exec);
+    hipGraphDestroy(s->stride_graph);
+    s->stride_exec = nullptr;
+    s->stride_graph = nullptr;
+  }
+  s->need_rebuild = true;
+}
+
+// ---- System packing ------------------------------------------------------
+
+static void check_index(int v, int n, const char *what) {
+  if (v < 0 || v >= n)
+    throw md_error(MD_ERR_INVALID_ARGUMENT, std::string(what) + " index " + std::to_string(v) + " out of range");
+}
+
+struct split_result {
+  std::vector<int4> waters;
+  std::vector<float2> water_d;
+  std::vector<std::vector<int>> cl_atoms;
+  std::vector<std::vector<int>> cl_cons;  // constraint indices
+  std::vector<int> free_atoms;
+  int max_atoms = 0;
+};
+
+// SETTLE triangles by f32-rounded equal sides, union-find for the rest
+// (src/engine/constraint-clusters.ts:93-249)
+static split_result split_constraints(const md_system_desc *d) {
+  int n = d->n_particles, nc = d->n_constraints;
+  split_result R;
+  std::vector<int> count(n, 0);
+  std::vector<char> live(nc, 0);
+  for (int c = 0; c < nc; ++c) {
+    int p1 = d->constraint_p1[c], p2 = d->constraint_p2[c];
+    if (d->masses[p1] == 0.0 && d->masses[p2] == 0.0) continue;
+    live[c] = 1;
+    count[p1]++;
+    count[p2]++;
+  }
+  std::vector<int> np(n, 0);
+  std::vector<int> pa(2 * (size_t)n, -1);
+  std::vector<float> pd(2 * (size_t)n, 0.0f);
+  for (int c = 0; c < nc; ++c) {
+    if (!live[c]) continue;
+    int p1 = d->constraint_p1[c], p2 = d->constraint_p2[c];
+    if (count[p1] == 2 && count[p2] == 2) {
+      float dist = (float)d->constraint_distance[c];
+      if (np[p1] < 2) pa[2 * p1 + np[p1]] = p2, pd[2 * p1 + np[p1]] = dist;
+      np[p1]++;
+      if (np[p2] < 2) pa[2 * p2 + np[p2]] = p1, pd[2 * p2 + np[p2]] = dist;
+      np[p2]++;
+    }
+  }
+  std::vector<char> settle(n, 0);
+  for (int p1 = 0; p1 < n; ++p1) {
+    if (np[p1] != 2) continue;
+    int p2 = pa[2 * p1], p3 = pa[2 * p1 + 1];
+    if (!(p1 < p2 && p1 < p3) || np[p2] != 2 || np[p3] != 2) continue;
+    float d23 = -1.0f;
+    for (int k = 0; k < 2; ++k)
+      if (pa[2 * p2 + k] == p3) d23 = pd[2 * p2 + k];
+    if (d23 < 0.0f) continue;
+    float d12 = pd[2 * p1], d13 = pd[2 * p1 + 1];
+    int4 w;
+    float2 dd;
+    if (d12 == d13)
+      w = make_int4(p1, p2, p3, 0), dd = make_float2(d12, d23);
+    else if (d12 == d23)
+      w = make_int4(p2, p1, p3, 0), dd = make_float2(d12, d13);
+    else if (d13 == d23)
+      w = make_int4(p3, p1, p2, 0), dd = make_float2(d13, d12);
+    else
+      continue;
+    R.waters.push_back(w);
+    R.water_d.push_back(dd);
+    settle[p1] = settle[p2] = settle[p3] = 1;
+  }
+  std::vector<int> parent(n);
+  std::iota(parent.begin(), parent.end(), 0);
+  auto find = [&](int x) {
+    while (parent[x] != x) x = parent[x] = parent[parent[x]];
+    return x;
+  };
+  std::vector<int> iterative;
+  for (int c = 0; c < nc; ++c) {
+    if (!live[c] || settle[d->constraint_p1[c]]) continue;
+    iterative.push_back(c);
+    int a = find(d->constraint_p1[c]), b = find(d->constraint_p2[c]);
+    if (a != b) parent[a] = b;
+  }
+  std::map<int, int> cluster_of_root;
+  std::vector<char> in_cluster(n, 0);
+  for (int c : iterative) {
+    int r = find(d->constraint_p1[c]);
+    auto it = cluster_of_root.find(r);
+    int k;
+    if (it == cluster_of_root.end()) {
+      k = (int)R.cl_atoms.size();
+      cluster_of_root[r] = k;
+      R.cl_atoms.emplace_back();
+      R.cl_cons.emplace_back();
+    } else {
+      k = it->second;
+    }
+    R.cl_cons[k].push_back(c);
+    for (int p : {d->constraint_p1[c], d->constraint_p2[c]})
+      if (!in_cluster[p]) {
+        in_cluster[p] = 1;
+        R.cl_atoms[k].push_back(p);
+      }
+  }
+  for (size_t k = 0; k < R.cl_atoms.size(); ++k) {
+    std::sort(R.cl_atoms[k].begin(), R.cl_atoms[k].end());
+    int na = (int)R.cl_atoms[k].size(), ncs = (int)R.cl_cons[k].size();
+    if (na > MD_MAX_CLUSTER_ATOMS || ncs > MD_MAX_CLUSTER_CONSTRAINTS)
+      throw md_error(MD_ERR_CLUSTER_TOO_LARGE,
+                     "a constraint cluster has " + std::to_string(na) + " atoms and " + std::to_string(ncs) +
+                         " constraints, beyond the 8 and 8 one thread holds; rebuild with constraints=HBonds");
+    R.max_atoms = std::max(R.max_atoms, na);
+  }
+  for (int i = 0; i < n; ++i)
+    if (!settle[i] && !in_cluster[i]) R.free_atoms.push_back(i);
+  return R;
+}
+
+template <typename T>
+static T *upload(md_system *s, const std::vector<T> &v) {
+  T *p = md_alloc<T>(s, std::max<size_t>(1, v.size()));
+  if (!v.empty()) MD_HIP(hipMemcpyAsync(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s->stream));
+  return p;
+}
+
+static int smooth_size(int minimum) {
+  for (int n = std::max(1, minimum);; ++n) {
+    int r = n;
+    for (int p : {2, 3, 5, 7})
+      while (r % p == 0) r /= p;
+    if (r == 1) return n;
+  }
+}
+
+static void create(md_system *s, const md_system_desc *d, const md_run_options *o) {
+  if (!d || !o) throw md_error(MD_ERR_INVALID_ARGUMENT, "null description or options");
+  int n = d->n_particles;
+  if (n <= 0) throw md_error(MD_ERR_INVALID_ARGUMENT, "a system needs at least one particle");
+  s->opt = *o;
+  if (s->opt.time_step <= 0) throw md_error(MD_ERR_INVALID_ARGUMENT, "time_step must be positive");
+  if (s->opt.constraint_tolerance <= 0) s->opt.constraint_tolerance = 1e-6;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    throw md_error(MD_ERR_NO_DEVICE, "no HIP device is visible; the engine has no CPU fallback");
+  if (o->device < 0 || o->device >= ndev) throw md_error(MD_ERR_INVALID_ARGUMENT, "no such HIP device");
+  s->device = o->device;
+  MD_HIP(hipSetDevice(s->device));
+  MD_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  s->n = n;
+  s->np = (n + 63) / 64 * 64;
+  s->masses.assign(d->masses, d->masses + n);
+  s->total_mass = 0;
+  int moving = 0;
+  for (double m : s->masses) {
+    if (m < 0) throw md_error(MD_ERR_INVALID_ARGUMENT, "negative mass");
+    s->total_mass += m;
+    moving += m > 0;
+  }
+  for (int c = 0; c < d->n_constraints; ++c) {
+    check_index(d->constraint_p1[c], n, "constraint");
+    check_index(d->constraint_p2[c], n, "constraint");
+  }
+  s->cmm_frequency = d->cmm_frequency;
+  s->n_constraints = d->n_constraints;
+  s->dof = 3 * moving - d->n_constraints - (d->cmm_frequency > 0 ? 3 : 0);
+
+  // nonbonded parameters and host constants
+  s->has_nonbonded = d->has_nonbonded != 0;
+  std::vector<float4> ap(s->np, make_float4(0, 0, 0, 0));
+  s->excl_start.assign(n + 1, 0);
+  if (s->has_nonbonded) {
+    s->cutoff = d->cutoff;
+    s->ewald_tol = d->ewald_tolerance;
+    if (s->cutoff <= 0) throw md_error(MD_ERR_INVALID_ARGUMENT, "cutoff must be positive");
+    s->alpha = d->alpha != 0.0 ? d->alpha : (1.0 / s->cutoff) * std::sqrt(-std::log(2.0 * s->ewald_tol));
+    s->dispersion = d->use_dispersion_correction != 0;
+    s->charge.assign(d->charge, d->charge + n);
+    s->sigma.assign(d->sigma, d->sigma + n);
+    s->epsilon.assign(d->epsilon, d->epsilon + n);
+    double sk = std::sqrt(MD_ONE_4PI_EPS0);
+    for (int i = 0; i < n; ++i) {
+      ap[i] = make_float4((float)(s->charge[i] * sk), (float)(0.5 * s->sigma[i]),
+                          (float)(2.0 * std::sqrt(s->epsilon[i])), (float)s->charge[i]);
+      s->q_total += s->charge[i];
+      s->q2_sum += s->charge[i] * s->charge[i];
+    }
+    // exclusion CSR, both directions (nonbonded.ts:159-182)
+    std::vector<std::vector<int>> ex(n);
+    for (int e = 0; e < d->n_exceptions; ++e) {
+      int a = d->exception_p1[e], b = d->exception_p2[e];
+      check_index(a, n, "exception");
+      check_index(b, n, "exception");
+      ex[a].push_back(b);
+      ex[b].push_back(a);
+    }
+    for (int i = 0; i < n; ++i) {
+      std::sort(ex[i].begin(), ex[i].end());
+      ex[i].erase(std::unique(ex[i].begin(), ex[i].end()), ex[i].end());
+      s->excl_start[i + 1] = s->excl_start[i] + (int)ex[i].size();
+      s->excl_list.insert(s->excl_list.end(), ex[i].begin(), ex[i].end());
+    }
+    // dispersion: grouped by distinct (sigma, epsilon) over the N(N+1)/2
+    // unordered pairs including i == j (nonbonded.ts:107-162)
+    if (s->dispersion) {
+      std::map<std::pair<double, double>, double> classes;
+      for (int i = 0; i < n; ++i) classes[{s->sigma[i], s->epsilon[i]}] += 1.0;
+      std::vector<std::pair<std::pair<double, double>, double>> cl(classes.begin(), classes.end());
+      double s6 = 0, s12 = 0;
+      for (size_t a = 0; a < cl.size(); ++a)
+        for (size_t b = a; b < cl.size(); ++b) {
+          double sig = 0.5 * (cl[a].first.first + cl[b].first.first);
+          double eps = std::sqrt(cl[a].first.second * cl[b].first.second);
+          double sig6 = std::pow(sig, 6);
+          double mult = a == b ? cl[a].second * (cl[a].second + 1) / 2 : cl[a].second * cl[b].second;
+          s6 += mult * eps * sig6;
+          s12 += mult * eps * sig6 * sig6;
+        }
+      double pairs = (double)n * (n + 1) / 2.0, rc = s->cutoff;
+      s->disp_coeff = 8.0 * M_PI * (double)n * n * ((s12 / pairs) / (9.0 * std::pow(rc, 9)) -
+                                                    (s6 / pairs) / (3.0 * std::pow(rc, 3)));
+    }
+  }
+  apply_box(s, d->box);
+
+  // PME mesh: run override, then the force's own, then from the box diagonal
+  if (s->has_nonbonded) {
+    for (int a = 0; a < 3; ++a) {
+      int g = o->mesh[a] > 0 ? o->mesh[a] : d->grid[a];
+      if (g <= 0) {
+        double L = s->box[4 * a];
+        int m = (int)std::ceil(2.0 * s->alpha * L / (3.0 * std::pow(s->ewald_tol, 0.2)));
+        g = smooth_size(std::max(6, m));
+      }
+      if (g < MD_PME_ORDER) throw md_error(MD_ERR_INVALID_ARGUMENT, "PME mesh smaller than the spline order");
+      s->grid[a] = g;
+    }
+  }
+
+  // bonded, exceptions, restraints: one list ordered by kind
+  std::vector<int4> ta;
+  std::vector<double4> tp;
+  for (int t = 0; t < d->n_bonds; ++t) {
+    check_index(d->bond_p1[t], n, "bond");
+    check_index(d->bond_p2[t], n, "bond");
+    ta.push_back(make_int4(d->bond_p1[t], d->bond_p2[t], 0, 0));
+    tp.push_back(make_double4(d->bond_length[t], d->bond_k[t], 0, 0));
+  }
+  for (int t = 0; t < d->n_angles; ++t) {
+    check_index(d->angle_p1[t], n, "angle");
+    check_index(d->angle_p2[t], n, "angle");
+    check_index(d->angle_p3[t], n, "angle");
+    ta.push_back(make_int4(d->angle_p1[t], d->angle_p2[t], d->angle_p3[t], 0));
+    tp.push_back(make_double4(d->angle_theta[t], d->angle_k[t], 0, 0));
+  }
+  for (int t = 0; t < d->n_torsions; ++t) {
+    for (const int32_t *p : {d->torsion_p1, d->torsion_p2, d->torsion_p3, d->torsion_p4}) check_index(p[t], n, "torsion");
+    ta.push_back(make_int4(d->torsion_p1[t], d->torsion_p2[t], d->torsion_p3[t], d->torsion_p4[t]));
+    tp.push_back(make_double4(d->torsion_phase[t], d->torsion_k[t], (double)d->torsion_periodicity[t], 0));
+  }
+  int nexc = s->has_nonbonded ? d->n_exceptions : 0;
+  for (int t = 0; t < nexc; ++t) {
+    int a = d->exception_p1[t], b = d->exception_p2[t];
+    ta.push_back(make_int4(a, b, 0, 0));
+    tp.push_back(make_double4(MD_ONE_4PI_EPS0 * d->charge[a] * d->charge[b],
+                              MD_ONE_4PI_EPS0 * d->exception_chargeprod[t], d->exception_sigma[t],
+                              d->exception_epsilon[t]));
+  }
+  std::vector<int64_t> rref(3 * (size_t)std::max(1, d->n_restraints), 0);
+  for (int t = 0; t < d->n_restraints; ++t) {
+    int p = d->restraint_particle[t];
+    check_index(p, n, "restraint");
+    ta.push_back(make_int4(p, 0, 0, 0));
+    tp.push_back(make_double4(d->restraint_k[t], 0, 0, 0));
+    rref[t] = fixed_checked(d->restraint_x0[t], "restraint x0");
+    rref[d->n_restraints + t] = fixed_checked(d->restraint_y0[t], "restraint y0");
+    rref[2 * (size_t)d->n_restraints + t] = fixed_checked(d->restraint_z0[t], "restraint z0");
+  }
+  s->n_bonds = d->n_bonds;
+  s->n_angles = d->n_angles;
+  s->n_torsions = d->n_torsions;
+  s->n_exceptions = nexc;
+  s->n_restraints = d->n_restraints;
+
+  // constraint work items
+  split_result R = split_constraints(d);
+  std::vector<int> ccl_atoms, ccl_na, ccl_nc;
+  std::vector<int4> ccl_cons;
+  for (size_t k = 0; k < R.cl_atoms.size(); ++k) {
+    const auto &at = R.cl_atoms[k];
+    for (int a = 0; a < MD_MAX_CLUSTER_ATOMS; ++a) ccl_atoms.push_back(a < (int)at.size() ? at[a] : 0);
+    ccl_na.push_back((int)at.size());
+    ccl_nc.push_back((int)R.cl_cons[k].size());
+    for (int c = 0; c < MD_MAX_CLUSTER_CONSTRAINTS; ++c) {
+      if (c < (int)R.cl_cons[k].size()) {
+        int ci = R.cl_cons[k][c];
+        int li = (int)(std::find(at.begin(), at.end(), d->constraint_p1[ci]) - at.begin());
+        int lj = (int)(std::find(at.begin(), at.end(), d->constraint_p2[ci]) - at.begin());
+        double dist = d->constraint_distance[ci];
+        int64_t bits;
+        std::memcpy(&bits, &dist, 8);
+        ccl_cons.push_back(make_int4(li, lj, (int)(uint32_t)(bits & 0xffffffffu), (int)(uint32_t)(bits >> 32)));
+      } else {
+        ccl_cons.push_back(make_int4(0, 0, 0, 0));
+      }
+    }
+  }
+  s->n_waters = (int)R.waters.size();
+  s->n_clusters_c = (int)R.cl_atoms.size();
+  s->n_free = (int)R.free_atoms.size();
+  s->max_cluster_atoms = R.max_atoms;
+
+  // device state
+  int np = s->np;
+  s->pos = md_alloc<int64_t>(s, 3 * (size_t)np);
+  s->vel = md_alloc<float>(s, 3 * (size_t)np);
+  s->resid = md_alloc<float>(s, 3 * (size_t)np);
+  s->force = md_alloc<int64_t>(s, 3 * (size_t)np);
+  std::vector<float> im(np, 0.0f), mf(np, 0.0f);
+  for (int i = 0; i < n; ++i) {
+    mf[i] = (float)s->masses[i];
+    im[i] = s->masses[i] == 0.0 ? 0.0f : (float)(1.0 / s->masses[i]);
+  }
+  s->inv_mass = upload(s, im);
+  s->mass_f = upload(s, mf);
+  s->atom_param = upload(s, ap);
+  s->energy = md_alloc<int64_t>(s, MD_E_SLOTS);
+  s->ctl = md_alloc<md_control>(s, 1);
+  s->term_atoms = upload(s, ta);
+  s->term_param = upload(s, tp);
+  s->restraint_ref = upload(s, rref);
+  s->water_atoms = upload(s, R.waters);
+  s->water_dist = upload(s, R.water_d);
+  s->ccl_atoms = upload(s, ccl_atoms);
+  s->ccl_natoms = upload(s, ccl_na);
+  s->ccl_ncons = upload(s, ccl_nc);
+  s->ccl_cons = upload(s, ccl_cons);
+  s->free_atoms = upload(s, R.free_atoms);
+  MD_HIP(hipStreamSynchronize(s->stream));
+  if (s->has_nonbonded) {
+    md_nb_setup(s);
+    md_pme_setup(s);
+  }
+  MD_HIP(hipStreamSynchronize(s->stream));
+}
+
+static void release(md_system *s) {
+  if (!s) return;
+  hipSetDevice(s->device);
+  if (s->stream) hipStreamSynchronize(s->stream);
+  if (s->stride_exec) hipGraphExecDestroy(s->stride_exec);
+  if (s->stride_graph) hipGraphDestroy(s->stride_graph);
+  if (s->nb_ext) md_nb_free(s);
+  if (s->pme_ext) md_pme_free(s);
+  for (void *p : s->allocations) hipFree(p);
+  if (s->stream) hipStreamDestroy(s->stream);
+  delete s;
+}
+
+// ---- steps and evaluations -------------------------------------------------
+
+static void launch_step(md_system *s, bool force_rebuild) {
+  md_step_prologue(s, true, force_rebuild);
+  if (s->has_nonbonded) {
+    md_nb_build(s, false);
+    md_nb_pairs(s, false, false);
+  }
+  md_bonded(s, false);
+  if (s->has_nonbonded) md_pme(s, false);
+  md_integrate(s);
+}
+
+static void check_control(md_system *s) {
+  md_control c;
+  MD_HIP(hipMemcpy(&c, s->ctl, sizeof c, hipMemcpyDeviceToHost));
+  if (c.overflow)
+    throw md_error(MD_ERR_LIST_OVERFLOW,
+                   "the cluster-pair list outgrew its capacity (the system compressed far beyond its starting "
+                   "density); recreate the system from the current state");
+}
+
+static void fault_tick(md_system *s, long count) {
+  if (s->fault_after < 0) return;
+  if (s->fault_after < count) {
+    s->fault_after = 0;
+    throw md_error(MD_ERR_FAULT_INJECTED, "fault injected by md_inject_fault");
+  }
+  s->fault_after -= count;
+}
+
+static void run_steps(md_system *s, int64_t steps) {
+  if (steps <= 0) return;
+  MD_HIP(hipSetDevice(s->device));
+  fault_tick(s, (long)steps);
+  MD_HIP(hipMemcpyAsync(&s->ctl->step_next, &s->steps_done, sizeof(int64_t), hipMemcpyHostToDevice, s->stream));
+  if (s->momentum_dirty) {
+    md_momentum(s);
+    s->momentum_dirty = false;
+  }
+  md_zero_forces(s, false);
+  int64_t left = steps;
+  if (s->need_rebuild) {
+    launch_step(s, true);
+    s->need_rebuild = false;
+    --left;
+  }
+  if (left >= STRIDE_STEPS && !s->stride_exec) {
+    MD_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < STRIDE_STEPS; ++k) launch_step(s, false);
+    MD_HIP(hipStreamEndCapture(s->stream, &s->stride_graph));
+    MD_HIP(hipGraphInstantiate(&s->stride_exec, s->stride_graph, nullptr, nullptr, 0));
+    s->stride_steps = STRIDE_STEPS;
+  }
+  while (s->stride_exec && left >= s->stride_steps) {
+    MD_HIP(hipGraphLaunch(s->stride_exec, s->stream));
+    left -= s->stride_steps;
+  }
+  for (; left > 0; --left) launch_step(s, false);
+  MD_HIP(hipStreamSynchronize(s->stream));
+  s->steps_done += steps;
+  check_control(s);
+}
+
+static void evaluate(md_system *s, double *forces, double *groups, double *total, bool census) {
+  MD_HIP(hipSetDevice(s->device));
+  fault_tick(s, 1);
+  md_zero_forces(s, true);
+  if (census) MD_HIP(hipMemsetAsync(&s->ctl->pair_census, 0, sizeof(int64_t), s->stream));
+  md_step_prologue(s, false, s->need_rebuild);
+  s->need_rebuild = false;
+  if (s->has_nonbonded) {
+    md_nb_build(s, false);
+    md_nb_pairs(s, true, census);
+  }
+  md_bonded(s, true);
+  if (s->has_nonbonded) md_pme(s, true);
+  int64_t e[MD_E_SLOTS];
+  MD_HIP(hipMemcpyAsync(e, s->energy, sizeof e, hipMemcpyDeviceToHost, s->stream));
+  std::vector<int64_t> f;
+  if (forces) {
+    f.resize(3 * (size_t)s->np);
+    MD_HIP(hipMemcpyAsync(f.data(), s->force, sizeof(int64_t) * f.size(), hipMemcpyDeviceToHost, s->stream));
+  }
+  MD_HIP(hipStreamSynchronize(s->stream));
+  check_control(s);
+  double g[MD_GROUP_COUNT] = {0};
+  g[MD_GROUP_BOND] = md_fixed_to_double(e[MD_E_BOND]);
+  g[MD_GROUP_ANGLE] = md_fixed_to_double(e[MD_E_ANGLE]);
+  g[MD_GROUP_TORSION] = md_fixed_to_double(e[MD_E_TORSION]);
+  g[MD_GROUP_RESTRAINT] = md_fixed_to_double(e[MD_E_RESTRAINT]);
+  if (s->has_nonbonded) {
+    double volume = box_volume(s->box);
+    g[MD_GROUP_DIRECT] = md_fixed_to_double(e[MD_E_PAIR]) + md_fixed_to_double(e[MD_E_EXCEPTION]) +
+                         (s->dispersion ? s->disp_coeff / volume : 0.0);
+    // self and net-charge terms (nonbonded.ts:594-617)
+    double self = -MD_ONE_4PI_EPS0 * s->alpha * s->q2_sum / std::sqrt(M_PI);
+    double net = -M_PI * s->q_total * s->q_total * MD_ONE_4PI_EPS0 / (2.0 * s->alpha * s->alpha * volume);
+    g[MD_GROUP_RECIPROCAL] = md_fixed_to_double(e[MD_E_RECIPROCAL]) + self + net;
+  }
+  if (groups) std::memcpy(groups, g, sizeof g);
+  if (total) {
+    *total = 0;
+    for (double v : g) *total += v;
+  }
+  if (forces)
+    for (int i = 0; i < s->n; ++i)
+      for (int d = 0; d < 3; ++d) forces[3 * i + d] = md_fixed_to_double(f[(size_t)d * s->np + i]);
+}
+
+// ---- probes ------------------------------------------------------------------
+
+__global__ void k_probe_fixed(const float *in, int n, int64_t *fixed, float *back) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int64_t f = md_to_fixed(in[i]);
+  fixed[i] = f;
+  back[i] = md_from_fixed(f);
+}
+
+__global__ void k_probe_thermostat(uint32_t seed, uint32_t step, int n, uint32_t *bits, float *normals) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  uint32_t a0, a1, b0, b1;
+  md_threefry2x32_20(2u * p, step, seed, MD_THERMOSTAT_DOMAIN, a0, a1);
+  md_threefry2x32_20(2u * p + 1u, step, seed, MD_THERMOSTAT_DOMAIN, b0, b1);
+  bits[4 * p] = a0;
+  bits[4 * p + 1] = a1;
+  bits[4 * p + 2] = b0;
+  bits[4 * p + 3] = b1;
+  const float two_pi = 6.283185307179586f;
+  float r1 = sqrtf(-2.0f * logf(md_open_unit(a0))), t1 = two_pi * md_open_unit(a1);
+  float r2 = sqrtf(-2.0f * logf(md_open_unit(b0))), t2 = two_pi * md_open_unit(b1);
+  normals[3 * p] = r1 * cosf(t1);
+  normals[3 * p + 1] = r1 * sinf(t1);
+  normals[3 * p + 2] = r2 * cosf(t2);
+}
+
+// ---- the C ABI -------------------------------------------------------------
+
+extern "C" {
+
+const char *md_last_error(void) { return g_last_error.c_str(); }
+
+const char *md_version(void) { return "mdcore 0.1 (gfx950)"; }
+
+md_err md_system_create(const md_system_desc *desc, const md_run_options *options, md_system **out) {
+  MD_API_BEGIN
+  if (!out) throw md_error(MD_ERR_INVALID_ARGUMENT, "null output handle");
+  *out = nullptr;
+  md_system *s = new md_system();
+  try {
+    create(s, desc, options);
+  } catch (...) {
+    release(s);
+    throw;
+  }
+  *out = s;
+  MD_API_END
+}
+
+void md_system_destroy(md_system *sys) { release(sys); }
+
+md_err md_set_positions(md_system *s, const double *x) {
+  MD_API_BEGIN
+  if (!s || !x) throw md_error(MD_ERR_INVALID_ARGUMENT, "null argument");
+  MD_HIP(hipSetDevice(s->device));
+  std::vector<int64_t> f(3 * (size_t)s->np, 0);
+  for (int i = 0; i < s->n; ++i)
+    for (int d = 0; d < 3; ++d) f[(size_t)d * s->np + i] = fixed_checked(x[3 * i + d], "position");
+  MD_HIP(hipMemcpyAsync(s->pos, f.data(), sizeof(int64_t) * f.size(), hipMemcpyHostToDevice, s->stream));
+  // setPositions clears the carried residual (src/gpu/loop.ts:819-833)
+  MD_HIP(hipMemsetAsync(s->resid, 0, sizeof(float) * 3 * s->np, s->stream));
+  MD_HIP(hipStreamSynchronize(s->stream));
+  s->need_rebuild = true;
+  MD_API_END
+}
+
+md_err md_set_velocities(md_system *s, const double *v) {
+  MD_API_BEGIN
+  if (!s || !v) throw md_error(MD_ERR_INVALID_ARGUMENT, "null argument");
+  MD_HIP(hipSetDevice(s->device));
+  std::vector<float> f(3 * (size_t)s->np, 0.0f);
+  for (int i = 0; i < s->n; ++i)
+    for (int d = 0; d < 3; ++d) f[(size_t)d * s->np + i] = (float)v[3 * i + d];
+  MD_HIP(hipMemcpy(s->vel, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice));
+  s->momentum_dirty = true;
+  MD_API_END
+}
+
+md_err md_get_positions(md_system *s, double *x) {
+  MD_API_BEGIN
+  if (!s || !x) throw md_error(MD_ERR_INVALID_ARGUMENT, "null argument");
+  MD_HIP(hipSetDevice(s->device));
+  std::vector<int64_t> f(3 * (size_t)s->np);
+  MD_HIP(hipMemcpy(f.data(), s->pos, sizeof(int64_t) * f.size(), hipMemcpyDeviceToHost));
+  for (int i = 0; i < s->n; ++i)
+    for (int d = 0; d < 3; ++d) x[3 * i + d] = md_fixed_to_double(f[(size_t)d * s->np + i]);
+  MD_API_END
+}
+
+md_err md_get_velocities(md_system *s, double *v) {
+  MD_API_BEGIN
+  if (!s || !v) throw md_error(MD_ERR_INVALID_ARGUMENT, "null argument");
+  MD_HIP(hipSetDevice(s->device));
+  std::vector<float> f(3 * (size_t)s->np);
+  MD_HIP(hipMemcpy(f.data(), s->vel, sizeof(float) * f.size(), hipMemcpyDeviceToHost));
+  for (int i = 0; i < s->n; ++i)
+    for (int d = 0; d < 3; ++d) v[3 * i + d] = f[(size_t)d * s->np + i];
+  MD_API_END
+}
+
+md_err md_set_box(md_system *s, const double box[9]) {
+  MD_API_BEGIN
+  if (!s || !box) throw md_error(MD_ERR_INVALID_ARGUMENT, "null argument");
+  apply_box(s, box);
+  MD_API_END
+}
+
+md_err md_get_box(md_system *s, double box[9]) {
+  MD_API_BEGIN
+  if (!s || !box) throw md_error(MD_ERR_INVALID_ARGUMENT, "null argument");
+  std::memcpy(box, s->box, sizeof s->box);
+  MD_API_END
+}
+
+md_err md_evaluate(md_system *s, double *forces, double *groups, double *total) {
+  MD_API_BEGIN
+  if (!s) throw md_error(MD_ERR_INVALID_ARGUMENT, "null handle");
+  evaluate(s, forces, groups, total, false);
+  MD_API_END
+}
+
+md_err md_run(md_system *s, int64_t steps) {
+  MD_API_BEGIN
+  if (!s || steps < 0) throw md_error(MD_ERR_INVALID_ARGUMENT, "bad arguments to md_run");
+  run_steps(s, steps);
+  MD_API_END
+}
+
+md_err md_get_step(md_system *s, int64_t *step) {
+  MD_API_BEGIN
+  if (!s || !step) throw md_error(MD_ERR_INVALID_ARGUMENT, "null argument");
+  *step = s->steps_done;
+  MD_API_END
+}
+
+md_err md_set_step(md_system *s, int64_t step) {
+  MD_API_BEGIN
+  if (!s || step < 0) throw md_error(MD_ERR_INVALID_ARGUMENT, "bad step");
+  s->steps_done = step;
+  s->momentum_dirty = true;
+  MD_API_END
+}
+
+md_err md_list_flags_get(md_system *s, md_list_flags *flags) {
+  MD_API_BEGIN
+  if (!s || !flags) throw md_error(MD_ERR_INVALID_ARGUMENT, "null argument");
+  std::memset(flags, 0, sizeof *flags);
+  if (!s->has_nonbonded) return MD_OK;
+  md_control c;
+  MD_HIP(hipMemcpy(&c, s->ctl, sizeof c, hipMemcpyDeviceToHost));
+  flags->rebuilds = c.rebuilds;
+  flags->tiles = md_nb_entries(s);
+  flags->tile_capacity = md_nb_capacity(s);
+  float w;
+  std::memcpy(&w, &c.max_disp2_bits[0], 4);
+  flags->worst_displacement = std::sqrt((double)w);
+  flags->overflowed = (int)c.overflow;
+  MD_API_END
+}
+
+md_err md_pair_census(md_system *s, int64_t *pairs, int64_t *tiles) {
+  MD_API_BEGIN
+  if (!s) throw md_error(MD_ERR_INVALID_ARGUMENT, "null handle");
+  evaluate(s, nullptr, nullptr, nullptr, true);
+  md_control c;
+  MD_HIP(hipMemcpy(&c, s->ctl, sizeof c, hipMemcpyDeviceToHost));
+  if (pairs) *pairs = c.pair_census;
+  if (tiles) *tiles = s->has_nonbonded ? md_nb_entries(s) : 0;
+  MD_API_END
+}
+
+md_err md_probe_fixed_point(const float *values, int32_t n, int64_t *fixed, float *back) {
+  MD_API_BEGIN
+  if (n < 0 || (n > 0 && (!values || !fixed || !back))) throw md_error(MD_ERR_INVALID_ARGUMENT, "bad probe arguments");
+  if (n == 0) return MD_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw md_error(MD_ERR_NO_DEVICE, "no HIP device");
+  float *din;
+  int64_t *dfix;
+  float *dback;
+  MD_HIP(hipMalloc(&din, sizeof(float) * n));
+  MD_HIP(hipMalloc(&dfix, sizeof(int64_t) * n));
+  MD_HIP(hipMalloc(&dback, sizeof(float) * n));
+  MD_HIP(hipMemcpy(din, values, sizeof(float) * n, hipMemcpyHostToDevice));
+  k_probe_fixed<<<(n + 255) / 256, 256>>>(din, n, dfix, dback);
+  MD_HIP(hipGetLastError());
+  MD_HIP(hipMemcpy(fixed, dfix, sizeof(int64_t) * n, hipMemcpyDeviceToHost));
+  MD_HIP(hipMemcpy(back, dback, sizeof(float) * n, hipMemcpyDeviceToHost));
+  hipFree(din);
+  hipFree(dfix);
+  hipFree(dback);
+  MD_API_END
+}
+
+md_err md_probe_thermostat(uint32_t seed, uint32_t step, int32_t n, uint32_t *bits, float *normals) {
+  MD_API_BEGIN
+  if (n < 0 || (n > 0 && (!bits || !normals))) throw md_error(MD_ERR_INVALID_ARGUMENT, "bad probe arguments");
+  if (n == 0) return MD_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw md_error(MD_ERR_NO_DEVICE, "no HIP device");
+  uint32_t *db;
+  float *dn;
+  MD_HIP(hipMalloc(&db, sizeof(uint32_t) * 4 * n));
+  MD_HIP(hipMalloc(&dn, sizeof(float) * 3 * n));
+  k_probe_thermostat<<<(n + 255) / 256, 256>>>(seed, step, n, db, dn);
+  MD_HIP(hipGetLastError());
+  MD_HIP(hipMemcpy(bits, db, sizeof(uint32_t) * 4 * n, hipMemcpyDeviceToHost));
+  MD_HIP(hipMemcpy(normals, dn, sizeof(float) * 3 * n, hipMemcpyDeviceToHost));
+  hipFree(db);
+  hipFree(dn);
+  MD_API_END
+}
+
+md_err md_inject_fault(md_system *s, int32_t after) {
+  MD_API_BEGIN
+  if (!s) throw md_error(MD_ERR_INVALID_ARGUMENT, "null handle");
+  s->fault_after = after;
+  MD_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,894 @@
+// nonbonded.hip -- direct-space LJ + erfc-Ewald for gfx950.
+//
+// Replaces the upstream WGSL passes `wrap`, `clear/assign/scan/scatter/
+// sort_cells` (src/gpu/kernels/cell-list.wgsl:139-331), the Verlet-list
+// build (verlet.wgsl:46-101, nonbonded-pairs.wgsl:417-497) and the pair
+// kernels `direct_pairs_list` / `direct_pairs` (nonbonded-pairs.wgsl:237-391).
+// The physics is the reference pair term, src/reference/forces/nonbonded.ts
+// :279-321: E = K q_i q_j erfc(alpha r)/r + 4 eps ((s/r)^12 - (s/r)^6), with
+// sigma the arithmetic and epsilon the geometric mean, for non-excluded pairs
+// inside the cutoff.
+//
+// Layout (designed for 64-lane wavefronts, not a translation of the WGSL):
+//   * Atoms are bucketed by an exact integer cell sort into cells ~0.3 nm wide
+//     whose order follows a 3D Hilbert curve; the sorted order is cut into
+//     clusters of 8 atoms and super-clusters of 64 (= one wavefront).
+//   * The list holds, per super-cluster S, the 8-atom j-clusters J (J >= the
+//     i-cluster, each unordered pair once) whose bounding boxes come within
+//     cutoff + skin, with an 8-bit mask of which of S's 8 i-clusters interact,
+//     the periodic shift of J into S's frame, and 8x8 exclusion masks.
+//   * The pair kernel gives lane l the pair (i = l/8, j = l%8) of each
+//     (i-cluster, j-cluster) block.  i-forces stay in registers across a
+//     segment of entries; j-forces are reduced across the 8 lanes sharing a j
+//     with cross-lane shuffles once per entry.  No LDS, no per-pair shuffle.
+//   * Positions reach the kernel as f32 offsets from the super-cluster's
+//     fixed-point frame origin, differenced exactly in Q32.32 first, so an
+//     absolute coordinate is never rounded (the reference's wrap-then-convert
+//     rule, fixed64.wgsl:166-216).
+//   * Every f32 partial sum covers one fixed list segment or entry and is
+//     converted to Q32.32 before it is added anywhere, so two identical runs
+//     are bit-identical whatever order the hardware schedules waves in.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "engine.hpp"
+
+// erfc(z) = exp(-z^2) P(t), t = 1/(1 + z/2): a degree-8 fit of erfcx(z) on
+// [0, 4] with minimax relative error ~3.4e-7 in f32 (fit_erfc.py beside this
+// file regenerates the coefficients).
+#define MD_ERFC_C0 -0.00045665048f
+#define MD_ERFC_C1 0.28968847f
+#define MD_ERFC_C2 0.22711307f
+#define MD_ERFC_C3 0.47317088f
+#define MD_ERFC_C4 -0.4014762f
+#define MD_ERFC_C5 1.010818f
+#define MD_ERFC_C6 -0.8866913f
+#define MD_ERFC_C7 0.33737373f
+#define MD_ERFC_C8 -0.049540002f
+
+__device__ __forceinline__ float md_erfcx_poly(float t) {
+  float p = MD_ERFC_C8;
+  p = fmaf(p, t, MD_ERFC_C7);
+  p = fmaf(p, t, MD_ERFC_C6);
+  p = fmaf(p, t, MD_ERFC_C5);
+  p = fmaf(p, t, MD_ERFC_C4);
+  p = fmaf(p, t, MD_ERFC_C3);
+  p = fmaf(p, t, MD_ERFC_C2);
+  p = fmaf(p, t, MD_ERFC_C1);
+  return fmaf(p, t, MD_ERFC_C0);
+}
+
+// ---------------------------------------------------------------------------
+// Build: fold, bucket, rank
+// ---------------------------------------------------------------------------
+
+struct md_build_ctx {
+  int n, np, ncells, gdim0, gdim1, gdim2;
+  int nclusters, nsuper, nslots;
+  int entry_capacity, segment_capacity;
+  float list_cutoff2;
+  float half_min_width;
+  float list_cutoff;
+  md_box box;
+};
+
+__device__ __forceinline__ bool md_build_gate(const md_control *ctl, int forced) {
+  return forced || ctl->rebuild_active;
+}
+
+// Fold each atom into the cell and bucket it (cell-list.wgsl:139-250).  The
+// fractional coordinate is computed in f64 from the exact fixed-point
+// position; the folding subtracts whole lattice vectors in Q32.32.
+__global__ void k_nb_assign(md_build_ctx c, const int64_t *__restrict__ pos, const int *__restrict__ cell_rank,
+                            int *cell_count, int *atom_cell, int64_t *folded, int64_t *build_pos,
+                            const md_control *ctl, int forced) {
+  if (!md_build_gate(ctl, forced)) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= c.n) return;
+  int64_t x = pos[i], y = pos[c.np + i], z = pos[2 * c.np + i];
+  const double s = 2.3283064365386963e-10;
+  double rx = (double)x * s, ry = (double)y * s, rz = (double)z * s;
+  const double *M = c.box.invd;
+  double fa = rx * M[0] + ry * M[3] + rz * M[6];
+  double fb = ry * M[4] + rz * M[7];
+  double fc = rz * M[8];
+  double na = floor(fa), nb = floor(fb), nc = floor(fc);
+  int64_t ia = (int64_t)na, ib = (int64_t)nb, ic = (int64_t)nc;
+  int64_t fx = x - ia * c.box.fa[0] - ib * c.box.fb[0] - ic * c.box.fc[0];
+  int64_t fy = y - ib * c.box.fb[1] - ic * c.box.fc[1];
+  int64_t fz = z - ic * c.box.fc[2];
+  double ua = fa - na, ub = fb - nb, uc = fc - nc;
+  int ga = min((int)(ua * c.gdim0), c.gdim0 - 1);
+  int gb = min((int)(ub * c.gdim1), c.gdim1 - 1);
+  int gc = min((int)(uc * c.gdim2), c.gdim2 - 1);
+  ga = max(ga, 0);
+  gb = max(gb, 0);
+  gc = max(gc, 0);
+  int key = cell_rank[(ga * c.gdim1 + gb) * c.gdim2 + gc];
+  atom_cell[i] = key;
+  atomicAdd(&cell_count[key], 1);
+  folded[i] = fx;
+  folded[c.np + i] = fy;
+  folded[2 * c.np + i] = fz;
+  build_pos[i] = x;
+  build_pos[c.np + i] = y;
+  build_pos[2 * c.np + i] = z;
+}
+
+// Exclusive scan by one workgroup of 1024 threads over `count` values; the
+// total lands at out[count].  `in` may alias `out`.  Zeroes `clear` if given.
+__device__ void md_block_scan(const int *in, int *out, int count, int *clear) {
+  __shared__ int partial[1024];
+  int t = threadIdx.x;
+  int per = (count + 1023) / 1024;
+  int begin = min(t * per, count), end = min(begin + per, count);
+  int sum = 0;
+  for (int k = begin; k < end; ++k) sum += in[k];
+  partial[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int v = t >= off ? partial[t - off] : 0;
+    __syncthreads();
+    partial[t] += v;
+    __syncthreads();
+  }
+  int run = partial[t] - sum;
+  for (int k = begin; k < end; ++k) {
+    int v = in[k];
+    out[k] = run;
+    run += v;
+    if (clear) clear[k] = 0;
+  }
+  if (t == 1023) out[count] = partial[1023];
+}
+
+__global__ void __launch_bounds__(1024) k_nb_scan_cells(int ncells, int *cell_count, int *cell_start,
+                                                        int *cell_cursor, md_control *ctl, int forced) {
+  if (!md_build_gate(ctl, forced)) return;
+  md_block_scan(cell_count, cell_start, ncells, nullptr);
+  __syncthreads();
+  for (int k = threadIdx.x; k < ncells; k += 1024) {
+    cell_count[k] = 0;
+    cell_cursor[k] = 0;
+  }
+  if (threadIdx.x == 0) {
+    ctl->rebuilds += 1;
+    ctl->max_disp2_bits[0] = 0;
+    ctl->cluster_reach_bits[0] = 0;
+    ctl->cluster_reach_bits[1] = 0;
+  }
+}
+
+__global__ void k_nb_scatter(int n, const int *__restrict__ atom_cell, const int *__restrict__ cell_start,
+                             int *cell_cursor, int *cell_member, const md_control *ctl, int forced) {
+  if (!md_build_gate(ctl, forced)) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int key = atom_cell[i];
+  int at = cell_start[key] + atomicAdd(&cell_cursor[key], 1);
+  cell_member[at] = i;
+}
+
+// Deterministic order inside a cell: rank by atom index over the cell's
+// members (the reference's `sort_cells`, cell-list.wgsl:318-331).
+__global__ void k_nb_rank(int n, const int *__restrict__ atom_cell, const int *__restrict__ cell_start,
+                          const int *__restrict__ cell_member, int *slot_atom, int *atom_slot,
+                          const md_control *ctl, int forced) {
+  if (!md_build_gate(ctl, forced)) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int key = atom_cell[i];
+  int b = cell_start[key], e = cell_start[key + 1];
+  int rank = 0;
+  for (int k = b; k < e; ++k) rank += cell_member[k] < i;
+  int slot = b + rank;
+  slot_atom[slot] = i;
+  atom_slot[i] = slot;
+}
+
+// Per 8-atom cluster: bounding box of the folded build positions.  Per
+// 64-atom super-cluster: a frame origin, and for each slot the fixed-point
+// offset that takes a current position to f32 coordinates in that frame.
+__global__ void k_nb_clusters(md_build_ctx c, const int *__restrict__ slot_atom,
+                              const int64_t *__restrict__ folded, const int64_t *__restrict__ build_pos,
+                              const float4 *__restrict__ atom_param, int64_t *cluster_center,
+                              float4 *cluster_half, int64_t *super_origin, int64_t *slot_offset,
+                              float2 *slot_lj, md_control *ctl, int forced) {
+  if (!md_build_gate(ctl, forced)) return;
+  // one thread per cluster; the 8 threads of a super-cluster cooperate
+  int k = blockIdx.x * blockDim.x + threadIdx.x;  // cluster index
+  int lane = threadIdx.x & 63;
+  bool live = k < c.nclusters;
+  int64_t lo[3] = {INT64_MAX, INT64_MAX, INT64_MAX}, hi[3] = {INT64_MIN, INT64_MIN, INT64_MIN};
+  if (live) {
+    for (int q = 0; q < MD_CLUSTER; ++q) {
+      int a = slot_atom[k * MD_CLUSTER + q];
+      if (a < 0) continue;
+      for (int d = 0; d < 3; ++d) {
+        int64_t v = folded[d * c.np + a];
+        lo[d] = min(lo[d], v);
+        hi[d] = max(hi[d], v);
+      }
+    }
+  }
+  int64_t cen[3];
+  float half[3];
+  for (int d = 0; d < 3; ++d) {
+    cen[d] = live && lo[d] <= hi[d] ? lo[d] + ((hi[d] - lo[d]) >> 1) : 0;
+    half[d] = live && lo[d] <= hi[d] ? md_from_fixed(hi[d] - lo[d]) * 0.5f : 0.0f;
+  }
+  float radius = sqrtf(half[0] * half[0] + half[1] * half[1] + half[2] * half[2]);
+  // reach of the super-cluster measured from its first cluster's centre
+  float reach = 0.0f;
+  {
+    int64_t c0[3];
+    for (int d = 0; d < 3; ++d) c0[d] = __shfl(cen[d], lane & ~7, 64);
+    float ddx = md_from_fixed(cen[0] - c0[0]), ddy = md_from_fixed(cen[1] - c0[1]),
+          ddz = md_from_fixed(cen[2] - c0[2]);
+    reach = live ? sqrtf(ddx * ddx + ddy * ddy + ddz * ddz) + radius : 0.0f;
+  }
+  if (live) {
+    for (int d = 0; d < 3; ++d) cluster_center[d * c.nclusters + k] = cen[d];
+    cluster_half[k] = make_float4(half[0], half[1], half[2], 0.0f);
+    atomicMax(&ctl->cluster_reach_bits[0], __float_as_uint(radius));
+    atomicMax(&ctl->cluster_reach_bits[1], __float_as_uint(reach));
+  }
+  // super-cluster frame: the bounding box centre over its 8 clusters
+  int64_t slo[3], shi[3];
+  for (int d = 0; d < 3; ++d) {
+    int64_t a = lo[d], b = hi[d];
+    for (int off = 1; off < 8; off <<= 1) {
+      a = min(a, (int64_t)__shfl_xor(a, off, 64));
+      b = max(b, (int64_t)__shfl_xor(b, off, 64));
+    }
+    slo[d] = a;
+    shi[d] = b;
+  }
+  int s = k / 8;
+  int64_t origin[3];
+  for (int d = 0; d < 3; ++d) origin[d] = slo[d] <= shi[d] ? slo[d] + ((shi[d] - slo[d]) >> 1) : 0;
+  if (live && (lane & 7) == 0 && s < c.nsuper)
+    for (int d = 0; d < 3; ++d) super_origin[d * c.nsuper + s] = origin[d];
+  if (!live) return;
+  for (int q = 0; q < MD_CLUSTER; ++q) {
+    int slot = k * MD_CLUSTER + q;
+    int a = slot_atom[slot];
+    if (a < 0) {
+      for (int d = 0; d < 3; ++d) slot_offset[d * c.nslots + slot] = 0;
+      slot_lj[slot] = make_float2(0.0f, 0.0f);
+      continue;
+    }
+    for (int d = 0; d < 3; ++d)
+      slot_offset[d * c.nslots + slot] = (build_pos[d * c.np + a] - folded[d * c.np + a]) + origin[d];
+    float4 p = atom_param[a];
+    slot_lj[slot] = make_float2(p.y, p.z);
+  }
+}
+
+// AABB distance test between cluster boxes separated by d (nm, f32)
+__device__ __forceinline__ bool md_boxes_within(float dx, float dy, float dz, float4 h1, float4 h2, float r2) {
+  float ex = fmaxf(fabsf(dx) - (h1.x + h2.x), 0.0f);
+  float ey = fmaxf(fabsf(dy) - (h1.y + h2.y), 0.0f);
+  float ez = fmaxf(fabsf(dz) - (h1.z + h2.z), 0.0f);
+  return ex * ex + ey * ey + ez * ez < r2;
+}
+
+// The candidate periodic images of cluster J relative to cluster c: the
+// minimum image of the centre difference, and -- when the box is narrow
+// compared with the list radius plus the cluster sizes -- its 26 neighbours.
+// Emits, in a fixed order, the image offsets (as lattice multiples added to
+// the reduced difference) that pass the box test.
+template <typename F>
+__device__ void md_for_each_image(const md_build_ctx &c, int64_t dx, int64_t dy, int64_t dz, bool enumerate,
+                                  F &&visit) {
+  md_min_image_fixed(dx, dy, dz, c.box);
+  if (!enumerate) {
+    visit(dx, dy, dz, 0);
+    return;
+  }
+  int code = 0;
+  for (int kc = -1; kc <= 1; ++kc)
+    for (int kb = -1; kb <= 1; ++kb)
+      for (int ka = -1; ka <= 1; ++ka, ++code) {
+        int64_t ex = dx + ka * c.box.fa[0] + kb * c.box.fb[0] + kc * c.box.fc[0];
+        int64_t ey = dy + kb * c.box.fb[1] + kc * c.box.fc[1];
+        int64_t ez = dz + kc * c.box.fc[2];
+        visit(ex, ey, ez, code - 13);  // 0 = the reduced image itself
+      }
+}
+
+struct md_list_ptrs {
+  const int64_t *cluster_center;
+  const float4 *cluster_half;
+  const int64_t *super_origin;
+  int *super_count;   // [nsuper+1] entries per super-cluster, then starts
+  int *super_segs;    // [nsuper+1] segments per super-cluster, then starts
+  int *entry_j;
+  int *entry_imask;
+  float4 *entry_shift;
+  int4 *segments;     // (S, begin, end, 0)
+  uint64_t *entry_excl;  // [capacity][8]
+};
+
+// Count (pass 0) or write (pass 1) the list entries of one super-cluster: one
+// wavefront per S, lanes over candidate J clusters, compacted with a
+// wavefront prefix count so the order is deterministic.
+// The entries one lane contributes for candidate cluster J: for each passing
+// periodic image, `emit(imask, self, shift)` in a fixed order.
+template <typename F>
+__device__ void md_candidate(const md_build_ctx &c, const md_list_ptrs &L, int first, int nlocal, int J,
+                             bool enumerate, const int64_t *cx, const int64_t *cy, const int64_t *cz,
+                             const float4 *ch, F &&emit) {
+  int64_t jx = L.cluster_center[J], jy = L.cluster_center[c.nclusters + J], jz = L.cluster_center[2 * c.nclusters + J];
+  float4 jh = L.cluster_half[J];
+  // the image is chosen against S's first cluster and shared by the entry
+  int64_t dx = jx - cx[0], dy = jy - cy[0], dz = jz - cz[0];
+  md_for_each_image(c, dx, dy, dz, enumerate, [&](int64_t ex, int64_t ey, int64_t ez, int code) {
+    // the lattice vector that moves J into this image
+    int64_t shx = ex - dx, shy = ey - dy, shz = ez - dz;
+    bool zero = shx == 0 && shy == 0 && shz == 0;
+    int imask = 0, self = 0;
+    for (int q = 0; q < nlocal; ++q) {
+      int I = first + q;
+      if (J < I) continue;
+      // a cluster with its own images: keep one of each +/- pair
+      if (J == I && !zero && code < 0) continue;
+      float ddx = md_from_fixed(jx + shx - cx[q]);
+      float ddy = md_from_fixed(jy + shy - cy[q]);
+      float ddz = md_from_fixed(jz + shz - cz[q]);
+      if (md_boxes_within(ddx, ddy, ddz, ch[q], jh, c.list_cutoff2)) {
+        imask |= 1 << q;
+        if (J == I && zero) self |= 1 << q;
+      }
+    }
+    if (imask) emit(imask, self, shx, shy, shz);
+  });
+}
+
+// Count (PASS 0) or write (PASS 1) the list entries of one super-cluster: one
+// wavefront per S, lanes over candidate J clusters, compacted with a
+// wavefront prefix count so the order is deterministic.
+template <int PASS>
+__global__ void __launch_bounds__(64) k_nb_list(md_build_ctx c, md_list_ptrs L, md_control *ctl, int forced) {
+  if (!md_build_gate(ctl, forced)) return;
+  int S = blockIdx.x;
+  int lane = threadIdx.x;
+  if (S >= c.nsuper) return;
+  float reach = __uint_as_float(ctl->cluster_reach_bits[0]) + __uint_as_float(ctl->cluster_reach_bits[1]);
+  bool enumerate = c.list_cutoff + reach > c.half_min_width;
+  int first = S * 8;
+  int nlocal = min(8, c.nclusters - first);
+  int64_t cx[8], cy[8], cz[8];
+  float4 ch[8];
+  for (int q = 0; q < 8; ++q) {
+    int k = min(first + q, c.nclusters - 1);
+    cx[q] = L.cluster_center[k];
+    cy[q] = L.cluster_center[c.nclusters + k];
+    cz[q] = L.cluster_center[2 * c.nclusters + k];
+    ch[q] = L.cluster_half[k];
+  }
+  int64_t ox = L.super_origin[S], oy = L.super_origin[c.nsuper + S], oz = L.super_origin[2 * c.nsuper + S];
+  int written = 0;
+  int base_out = PASS == 1 ? L.super_count[S] : 0;
+  for (int jb = first; jb < c.nclusters; jb += 64) {
+    int J = jb + lane;
+    bool valid = J < c.nclusters;
+    int count = 0;
+    if (valid)
+      md_candidate(c, L, first, nlocal, J, enumerate, cx, cy, cz, ch,
+                   [&](int, int, int64_t, int64_t, int64_t) { ++count; });
+    int incl = count;
+    for (int off = 1; off < 64; off <<= 1) {
+      int v = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += v;
+    }
+    if (PASS == 1 && valid && count) {
+      int at = base_out + written + incl - count;
+      int SJ = J / 8;
+      int64_t jox = L.super_origin[SJ], joy = L.super_origin[c.nsuper + SJ], joz = L.super_origin[2 * c.nsuper + SJ];
+      md_candidate(c, L, first, nlocal, J, enumerate, cx, cy, cz, ch,
+                   [&](int imask, int self, int64_t shx, int64_t shy, int64_t shz) {
+                     if (at < c.entry_capacity) {
+                       L.entry_j[at] = J;
+                       L.entry_imask[at] = imask | (self << 8);
+                       // J's frame origin, moved into this image, in S's frame
+                       L.entry_shift[at] = make_float4(md_from_fixed(jox + shx - ox), md_from_fixed(joy + shy - oy),
+                                                       md_from_fixed(joz + shz - oz), 0.0f);
+                     }
+                     ++at;
+                   });
+    }
+    written += __shfl(incl, 63, 64);
+  }
+  if (PASS == 0 && lane == 0) L.super_count[S] = written;
+}
+
+// Starts of every super-cluster's entries and segments, and the totals.
+__global__ void __launch_bounds__(1024) k_nb_list_scan(int nsuper, md_list_ptrs L, int entry_capacity,
+                                                       int segment_capacity, md_control *ctl, int *totals,
+                                                       int forced) {
+  if (!md_build_gate(ctl, forced)) return;
+  for (int s = threadIdx.x; s < nsuper; s += 1024) L.super_segs[s] = (L.super_count[s] + MD_SEGMENT - 1) / MD_SEGMENT;
+  __syncthreads();
+  md_block_scan(L.super_count, L.super_count, nsuper, nullptr);
+  __syncthreads();
+  md_block_scan(L.super_segs, L.super_segs, nsuper, nullptr);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int entries = L.super_count[nsuper], segs = L.super_segs[nsuper];
+    if (entries > entry_capacity || segs > segment_capacity) ctl->overflow = 1;
+    totals[0] = min(entries, entry_capacity);
+    totals[1] = min(segs, segment_capacity);
+  }
+}
+
+__global__ void k_nb_segments(int nsuper, md_list_ptrs L, int entry_capacity, int segment_capacity,
+                              const md_control *ctl, int forced) {
+  if (!md_build_gate(ctl, forced)) return;
+  int S = blockIdx.x * blockDim.x + threadIdx.x;
+  if (S >= nsuper) return;
+  int b = L.super_count[S], e = L.super_count[S + 1];
+  int sg = L.super_segs[S];
+  for (int at = b; at < e; at += MD_SEGMENT, ++sg) {
+    if (sg >= segment_capacity) break;
+    L.segments[sg] = make_int4(S, min(at, entry_capacity), min(min(at + MD_SEGMENT, e), entry_capacity), 0);
+  }
+}
+
+// Exclusion masks: one wavefront per entry, lane l = (i = l/8, j = l%8); for
+// each of S's 8 i-clusters a 64-bit mask of the pairs that interact.  Clears
+// padding, the lower triangle of a cluster with its own zero image, and every
+// excluded pair (the CSR holds both directions; nonbonded.ts:159-182).
+__global__ void __launch_bounds__(64) k_nb_masks(md_build_ctx c, md_list_ptrs L, const int *__restrict__ slot_atom,
+                                                 const int *__restrict__ atom_slot, const int *__restrict__ excl_start,
+                                                 const int *__restrict__ excl_list, const int *totals,
+                                                 const md_control *ctl, int forced) {
+  if (!md_build_gate(ctl, forced)) return;
+  int e = blockIdx.x;
+  if (e >= totals[0]) return;
+  int lane = threadIdx.x, il = lane >> 3, jl = lane & 7;
+  // which super-cluster owns entry e: binary search the starts
+  int lo = 0, hi = c.nsuper;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (L.super_count[mid] <= e) lo = mid; else hi = mid;
+  }
+  int S = lo;
+  int J = L.entry_j[e];
+  int meta = L.entry_imask[e];
+  int jslot = J * MD_CLUSTER + jl;
+  int ja = slot_atom[jslot];
+  for (int q = 0; q < 8; ++q) {
+    int I = S * 8 + q;
+    int islot = I * MD_CLUSTER + il;
+    int ia = I < c.nclusters ? slot_atom[islot] : -1;
+    bool ok = ((meta >> q) & 1) && ia >= 0 && ja >= 0;
+    if (ok && ((meta >> (8 + q)) & 1)) ok = jl > il;  // same cluster, zero image
+    if (ok) {
+      for (int k = excl_start[ia]; k < excl_start[ia + 1]; ++k)
+        if (excl_list[k] == ja) {
+          ok = false;
+          break;
+        }
+    }
+    uint64_t m = __ballot(ok);
+    if (lane == 0) L.entry_excl[(size_t)e * 8 + q] = m;
+  }
+}
+
+// Per step: each slot's position as f32 in its super-cluster's frame.
+__global__ void k_nb_pack(int nslots, int np, const int *__restrict__ slot_atom, const int64_t *__restrict__ pos,
+                          const int64_t *__restrict__ slot_offset, const float4 *__restrict__ atom_param,
+                          float4 *slot_xyzq) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslots) return;
+  int a = slot_atom[s];
+  if (a < 0) {
+    // far away and uncharged; every pair with it is masked anyway
+    slot_xyzq[s] = make_float4(1.0e4f + (float)(s & 63), 1.0e4f, 1.0e4f, 0.0f);
+    return;
+  }
+  float x = md_from_fixed(pos[a] - slot_offset[s]);
+  float y = md_from_fixed(pos[np + a] - slot_offset[nslots + s]);
+  float z = md_from_fixed(pos[2 * np + a] - slot_offset[2 * nslots + s]);
+  slot_xyzq[s] = make_float4(x, y, z, atom_param[a].x);
+}
+
+// ---------------------------------------------------------------------------
+// The pair kernel
+// ---------------------------------------------------------------------------
+
+struct md_pair_args {
+  const int4 *segments;
+  const int *entry_j;
+  const int *entry_imask;
+  const float4 *entry_shift;
+  const uint64_t *entry_excl;
+  const float4 *slot_xyzq;
+  const float2 *slot_lj;
+  const int *slot_atom;
+  const int *totals;
+  int64_t *force;
+  int64_t *energy;
+  int64_t *census;
+  int np, nclusters;
+  float cutoff2, alpha, beta, exp_scale;
+};
+
+// one pair from lane's i atom to the entry's j atom; returns the scalar
+// F/r factor (so F_i = -fs * d) and accumulates the pair energy
+template <bool ENERGY>
+__device__ __forceinline__ float md_pair(float dx, float dy, float dz, float qi, float si, float ei, float qj,
+                                         float sj, float ej, bool allowed, const md_pair_args &A, float &en,
+                                         int &count) {
+  float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+  bool inc = allowed && r2 < A.cutoff2;
+  float rinv = __builtin_amdgcn_rsqf(r2);
+  float rinv2 = rinv * rinv;
+  float r = r2 * rinv;
+  float ex = __builtin_amdgcn_exp2f(A.exp_scale * r2);  // exp(-alpha^2 r^2)
+  float t = __builtin_amdgcn_rcpf(fmaf(0.5f * A.alpha, r, 1.0f));
+  float erfc_v = ex * md_erfcx_poly(t);
+  float qq = qi * qj;
+  float fc = qq * fmaf(erfc_v, rinv, A.beta * ex) * rinv2;
+  float sig = si + sj;
+  float eps = ei * ej;  // 4 eps_ij
+  float s2 = sig * sig * rinv2;
+  float s6 = s2 * s2 * s2;
+  float s12 = s6 * s6;
+  float flj = eps * fmaf(12.0f, s12, -6.0f * s6) * rinv2;
+  float fs = inc ? fc + flj : 0.0f;
+  if (ENERGY) {
+    float e = fmaf(qq * erfc_v, rinv, eps * (s12 - s6));
+    en += inc ? e : 0.0f;
+    count += inc ? 1 : 0;
+  }
+  return fs;
+}
+
+template <bool ENERGY>
+__global__ void __launch_bounds__(64) k_nb_pairs(md_pair_args A) {
+  int seg = blockIdx.x;
+  if (seg >= A.totals[1]) return;
+  int4 sg = A.segments[seg];
+  int S = sg.x, eb = sg.y, ee = sg.z;
+  int lane = threadIdx.x, il = lane >> 3, jl = lane & 7;
+  // i data: lane's atom in each of S's 8 clusters (8 clusters x 8 atoms = 64
+  // slots; lane il*8+jl reads slot S*64 + q*8 + il)
+  float xi[8], yi[8], zi[8], qi[8], si[8], ei[8];
+  float fx[8], fy[8], fz[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    int slot = S * MD_SUPER + q * MD_CLUSTER + il;
+    float4 p = A.slot_xyzq[slot];
+    float2 lj = A.slot_lj[slot];
+    xi[q] = p.x;
+    yi[q] = p.y;
+    zi[q] = p.z;
+    qi[q] = p.w;
+    si[q] = lj.x;
+    ei[q] = lj.y;
+    fx[q] = fy[q] = fz[q] = 0.0f;
+  }
+  float en = 0.0f;
+  int count = 0;
+  for (int e = eb; e < ee; ++e) {
+    int J = A.entry_j[e];
+    int meta = A.entry_imask[e];
+    float4 sh = A.entry_shift[e];
+    int jslot = J * MD_CLUSTER + jl;
+    float4 pj = A.slot_xyzq[jslot];
+    float2 lj = A.slot_lj[jslot];
+    float xj = pj.x + sh.x, yj = pj.y + sh.y, zj = pj.z + sh.z;
+    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+    const uint64_t *ex = A.entry_excl + (size_t)e * 8;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      if ((meta >> q) & 1) {
+        uint64_t m = ex[q];
+        bool allowed = (m >> lane) & 1;
+        float dx = xj - xi[q], dy = yj - yi[q], dz = zj - zi[q];
+        float fs = md_pair<ENERGY>(dx, dy, dz, qi[q], si[q], ei[q], pj.w, lj.x, lj.y, allowed, A, en, count);
+        fx[q] = fmaf(-fs, dx, fx[q]);
+        fy[q] = fmaf(-fs, dy, fy[q]);
+        fz[q] = fmaf(-fs, dz, fz[q]);
+        gx = fmaf(fs, dx, gx);
+        gy = fmaf(fs, dy, gy);
+        gz = fmaf(fs, dz, gz);
+      }
+    }
+    // j force: sum the 8 lanes that share j (lane%8), fixed order
+#pragma unroll
+    for (int off = 8; off < 64; off <<= 1) {
+      gx += __shfl_xor(gx, off, 64);
+      gy += __shfl_xor(gy, off, 64);
+      gz += __shfl_xor(gz, off, 64);
+    }
+    if (lane < 8) {
+      int a = A.slot_atom[J * MD_CLUSTER + lane];
+      if (a >= 0) {
+        md_atomic_add_i64(&A.force[a], md_to_fixed(gx));
+        md_atomic_add_i64(&A.force[A.np + a], md_to_fixed(gy));
+        md_atomic_add_i64(&A.force[2 * A.np + a], md_to_fixed(gz));
+      }
+    }
+  }
+  // i forces: sum the 8 lanes that share i (same il), fixed order
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    float x = fx[q], y = fy[q], z = fz[q];
+    for (int off = 1; off < 8; off <<= 1) {
+      x += __shfl_xor(x, off, 64);
+      y += __shfl_xor(y, off, 64);
+      z += __shfl_xor(z, off, 64);
+    }
+    if (jl == 0) {
+      int I = S * 8 + q;
+      int a = I < A.nclusters ? A.slot_atom[I * MD_CLUSTER + il] : -1;
+      if (a >= 0) {
+        md_atomic_add_i64(&A.force[a], md_to_fixed(x));
+        md_atomic_add_i64(&A.force[A.np + a], md_to_fixed(y));
+        md_atomic_add_i64(&A.force[2 * A.np + a], md_to_fixed(z));
+      }
+    }
+  }
+  if (ENERGY) {
+    int64_t ef = md_wave_sum_i64(md_to_fixed(en));
+    int64_t cf = md_wave_sum_i64((int64_t)count);
+    if (lane == 0) {
+      md_atomic_add_i64(A.energy + MD_E_PAIR, ef);
+      md_atomic_add_i64(A.census, cf);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+// Skilling's transpose form of the Hilbert index (J. Skilling, "Programming
+// the Hilbert curve", AIP Conf. Proc. 707, 381 (2004)).
+static uint64_t hilbert_index(uint32_t x, uint32_t y, uint32_t z, int bits) {
+  uint32_t X[3] = {x, y, z};
+  uint32_t M = 1u << (bits - 1);
+  for (uint32_t Q = M; Q > 1; Q >>= 1) {
+    uint32_t P = Q - 1;
+    for (int i = 0; i < 3; i++) {
+      if (X[i] & Q) {
+        X[0] ^= P;
+      } else {
+        uint32_t t = (X[0] ^ X[i]) & P;
+        X[0] ^= t;
+        X[i] ^= t;
+      }
+    }
+  }
+  for (int i = 1; i < 3; i++) X[i] ^= X[i - 1];
+  uint32_t t = 0;
+  for (uint32_t Q = M; Q > 1; Q >>= 1)
+    if (X[2] & Q) t ^= Q - 1;
+  for (int i = 0; i < 3; i++) X[i] ^= t;
+  uint64_t h = 0;
+  for (int b = bits - 1; b >= 0; --b)
+    for (int i = 0; i < 3; i++) h = (h << 1) | ((X[i] >> b) & 1u);
+  return h;
+}
+
+struct md_nb_buffers {
+  int64_t *cluster_center = nullptr, *super_origin = nullptr;
+  int *super_count = nullptr, *super_segs = nullptr;
+  int *entry_j = nullptr, *entry_imask = nullptr;
+  float4 *entry_shift = nullptr;
+  int4 *segments = nullptr;
+  uint64_t *entry_excl = nullptr;
+  int *totals = nullptr;
+  int nsuper = 0;
+  int entry_capacity = 0, segment_capacity = 0;
+  float half_min_width = 0, list_cutoff = 0;
+};
+
+static md_nb_buffers &nbuf(md_system *s) {
+  static_assert(sizeof(md_nb_buffers) > 0, "");
+  return *reinterpret_cast<md_nb_buffers *>(s->nb_ext);
+}
+
+void md_nb_setup(md_system *s) {
+  auto &nb = s->nb;
+  s->nb_ext = new md_nb_buffers();
+  auto &B = nbuf(s);
+  nb.nclusters = (s->n + MD_CLUSTER - 1) / MD_CLUSTER;
+  B.nsuper = (nb.nclusters + 7) / 8;
+  nb.nslots = B.nsuper * MD_SUPER;  // whole super-clusters
+  double skin = s->opt.verlet_skin < 0 ? 0.15 : s->opt.verlet_skin;
+  double rlist = s->cutoff + skin;
+  nb.cutoff2 = (float)(s->cutoff * s->cutoff);
+  nb.list_cutoff2 = (float)(rlist * rlist);
+  nb.half_skin2 = (float)(0.25 * skin * skin);
+  B.list_cutoff = (float)rlist;
+  // sorting cells ~0.3 nm along each fractional axis
+  double widths[3];
+  {
+    const double *b = s->box;
+    double v = std::fabs(b[0] * (b[4] * b[8] - b[5] * b[7]) - b[1] * (b[3] * b[8] - b[5] * b[6]) +
+                         b[2] * (b[3] * b[7] - b[4] * b[6]));
+    auto cross_norm = [](const double *u, const double *w) {
+      double x = u[1] * w[2] - u[2] * w[1], y = u[2] * w[0] - u[0] * w[2], z = u[0] * w[1] - u[1] * w[0];
+      return std::sqrt(x * x + y * y + z * z);
+    };
+    widths[0] = v / cross_norm(b + 3, b + 6);
+    widths[1] = v / cross_norm(b + 6, b);
+    widths[2] = v / cross_norm(b, b + 3);
+  }
+  B.half_min_width = (float)(0.5 * std::min(widths[0], std::min(widths[1], widths[2])));
+  for (int d = 0; d < 3; ++d) nb.gdim[d] = std::max(1, (int)std::floor(widths[d] / 0.3));
+  nb.ncells = nb.gdim[0] * nb.gdim[1] * nb.gdim[2];
+  int bits = 1;
+  while ((1 << bits) < std::max(nb.gdim[0], std::max(nb.gdim[1], nb.gdim[2]))) ++bits;
+  std::vector<std::pair<uint64_t, int>> order(nb.ncells);
+  for (int a = 0; a < nb.gdim[0]; ++a)
+    for (int b = 0; b < nb.gdim[1]; ++b)
+      for (int c = 0; c < nb.gdim[2]; ++c) {
+        int cell = (a * nb.gdim[1] + b) * nb.gdim[2] + c;
+        order[cell] = {hilbert_index(a, b, c, bits), cell};
+      }
+  std::sort(order.begin(), order.end());
+  std::vector<int> rank(nb.ncells);
+  for (int k = 0; k < nb.ncells; ++k) rank[order[k].second] = k;
+
+  int np = s->np;
+  s->cell_rank = md_alloc<int>(s, nb.ncells);
+  MD_HIP(hipMemcpyAsync(s->cell_rank, rank.data(), sizeof(int) * nb.ncells, hipMemcpyHostToDevice, s->stream));
+  s->cell_count = md_alloc<int>(s, nb.ncells);
+  s->cell_cursor = md_alloc<int>(s, nb.ncells);
+  s->cell_start = md_alloc<int>(s, nb.ncells + 1);
+  s->atom_cell = md_alloc<int>(s, np);
+  s->cell_member = md_alloc<int>(s, np);
+  s->slot_atom = md_alloc<int>(s, nb.nslots);
+  std::vector<int> pad(nb.nslots, -1);
+  MD_HIP(hipMemcpyAsync(s->slot_atom, pad.data(), sizeof(int) * nb.nslots, hipMemcpyHostToDevice, s->stream));
+  MD_HIP(hipStreamSynchronize(s->stream));
+  s->atom_slot = md_alloc<int>(s, np);
+  s->atom_folded = md_alloc<int64_t>(s, 3 * (size_t)np);
+  s->build_pos = md_alloc<int64_t>(s, 3 * (size_t)np);
+  s->slot_offset = md_alloc<int64_t>(s, 3 * (size_t)nb.nslots);
+  s->slot_lj = md_alloc<float2>(s, nb.nslots);
+  s->slot_xyzq = md_alloc<float4>(s, nb.nslots);
+  s->cluster_half = md_alloc<float4>(s, nb.nclusters);
+  B.cluster_center = md_alloc<int64_t>(s, 3 * (size_t)nb.nclusters);
+  B.super_origin = md_alloc<int64_t>(s, 3 * (size_t)B.nsuper);
+  B.super_count = md_alloc<int>(s, B.nsuper + 1);
+  B.super_segs = md_alloc<int>(s, B.nsuper + 1);
+  B.totals = md_alloc<int>(s, 4);
+  // list capacity: the cluster pairs within the list radius plus the cluster
+  // extents at the system's density, doubled for headroom
+  double volume = std::fabs(s->box[0] * s->box[4] * s->box[8]);
+  double density = s->n / volume;
+  double ext = std::cbrt(MD_CLUSTER / std::max(density, 1.0));
+  double reach = rlist + 2.0 * ext;
+  double per_super = 8.0 * (4.0 / 3.0 * M_PI * reach * reach * reach) * density / MD_CLUSTER;
+  long cap = (long)(2.0 * B.nsuper * std::min(per_super, (double)nb.nclusters)) + 1024;
+  cap = std::min(cap, (long)B.nsuper * (long)nb.nclusters * 27 + 64);
+  B.entry_capacity = (int)cap;
+  B.segment_capacity = B.entry_capacity / MD_SEGMENT + B.nsuper + 64;
+  B.entry_j = md_alloc<int>(s, B.entry_capacity);
+  B.entry_imask = md_alloc<int>(s, B.entry_capacity);
+  B.entry_shift = md_alloc<float4>(s, B.entry_capacity);
+  B.entry_excl = md_alloc<uint64_t>(s, (size_t)B.entry_capacity * 8);
+  B.segments = md_alloc<int4>(s, B.segment_capacity);
+  s->excl_start_d = md_alloc<int>(s, s->n + 1);
+  s->excl_list_d = md_alloc<int>(s, std::max<size_t>(1, s->excl_list.size()));
+  MD_HIP(hipMemcpyAsync(s->excl_start_d, s->excl_start.data(), sizeof(int) * (s->n + 1), hipMemcpyHostToDevice,
+                        s->stream));
+  if (!s->excl_list.empty())
+    MD_HIP(hipMemcpyAsync(s->excl_list_d, s->excl_list.data(), sizeof(int) * s->excl_list.size(),
+                          hipMemcpyHostToDevice, s->stream));
+  MD_HIP(hipStreamSynchronize(s->stream));
+}
+
+void md_nb_free(md_system *s) {
+  delete reinterpret_cast<md_nb_buffers *>(s->nb_ext);
+  s->nb_ext = nullptr;
+}
+
+static md_build_ctx build_ctx(md_system *s) {
+  auto &B = nbuf(s);
+  md_build_ctx c{};
+  c.n = s->n;
+  c.np = s->np;
+  c.ncells = s->nb.ncells;
+  c.gdim0 = s->nb.gdim[0];
+  c.gdim1 = s->nb.gdim[1];
+  c.gdim2 = s->nb.gdim[2];
+  c.nclusters = s->nb.nclusters;
+  c.nsuper = B.nsuper;
+  c.nslots = s->nb.nslots;
+  c.entry_capacity = B.entry_capacity;
+  c.segment_capacity = B.segment_capacity;
+  c.list_cutoff2 = s->nb.list_cutoff2;
+  c.list_cutoff = B.list_cutoff;
+  c.half_min_width = B.half_min_width;
+  c.box = s->dbox;
+  return c;
+}
+
+static md_list_ptrs list_ptrs(md_system *s) {
+  auto &B = nbuf(s);
+  md_list_ptrs L;
+  L.cluster_center = B.cluster_center;
+  L.cluster_half = s->cluster_half;
+  L.super_origin = B.super_origin;
+  L.super_count = B.super_count;
+  L.super_segs = B.super_segs;
+  L.entry_j = B.entry_j;
+  L.entry_imask = B.entry_imask;
+  L.entry_shift = B.entry_shift;
+  L.segments = B.segments;
+  L.entry_excl = B.entry_excl;
+  return L;
+}
+
+void md_nb_build(md_system *s, bool forced) {
+  auto &B = nbuf(s);
+  md_build_ctx c = build_ctx(s);
+  md_list_ptrs L = list_ptrs(s);
+  int f = forced ? 1 : 0;
+  hipStream_t st = s->stream;
+  int n = s->n;
+  int g = (n + 255) / 256;
+  k_nb_assign<<<g, 256, 0, st>>>(c, s->pos, s->cell_rank, s->cell_count, s->atom_cell, s->atom_folded,
+                                 s->build_pos, s->ctl, f);
+  k_nb_scan_cells<<<1, 1024, 0, st>>>(c.ncells, s->cell_count, s->cell_start, s->cell_cursor, s->ctl, f);
+  k_nb_scatter<<<g, 256, 0, st>>>(n, s->atom_cell, s->cell_start, s->cell_cursor, s->cell_member, s->ctl, f);
+  k_nb_rank<<<g, 256, 0, st>>>(n, s->atom_cell, s->cell_start, s->cell_member, s->slot_atom, s->atom_slot, s->ctl, f);
+  int gc = (c.nclusters + 63) / 64;
+  k_nb_clusters<<<gc, 64, 0, st>>>(c, s->slot_atom, s->atom_folded, s->build_pos, s->atom_param, B.cluster_center,
+                                   s->cluster_half, B.super_origin, s->slot_offset, s->slot_lj, s->ctl, f);
+  k_nb_list<0><<<B.nsuper, 64, 0, st>>>(c, L, s->ctl, f);
+  k_nb_list_scan<<<1, 1024, 0, st>>>(B.nsuper, L, B.entry_capacity, B.segment_capacity, s->ctl, B.totals, f);
+  k_nb_list<1><<<B.nsuper, 64, 0, st>>>(c, L, s->ctl, f);
+  k_nb_segments<<<(B.nsuper + 63) / 64, 64, 0, st>>>(B.nsuper, L, B.entry_capacity, B.segment_capacity, s->ctl, f);
+  k_nb_masks<<<B.entry_capacity, 64, 0, st>>>(c, L, s->slot_atom, s->atom_slot, s->excl_start_d, s->excl_list_d,
+                                              B.totals, s->ctl, f);
+  MD_HIP(hipGetLastError());
+}
+
+void md_nb_pairs(md_system *s, bool energy, bool census) {
+  auto &B = nbuf(s);
+  hipStream_t st = s->stream;
+  int gs = (s->nb.nslots + 255) / 256;
+  k_nb_pack<<<gs, 256, 0, st>>>(s->nb.nslots, s->np, s->slot_atom, s->pos, s->slot_offset, s->atom_param,
+                                s->slot_xyzq);
+  md_pair_args A;
+  A.segments = B.segments;
+  A.entry_j = B.entry_j;
+  A.entry_imask = B.entry_imask;
+  A.entry_shift = B.entry_shift;
+  A.entry_excl = B.entry_excl;
+  A.slot_xyzq = s->slot_xyzq;
+  A.slot_lj = s->slot_lj;
+  A.slot_atom = s->slot_atom;
+  A.totals = B.totals;
+  A.force = s->force;
+  A.energy = s->energy;
+  A.census = &s->ctl->pair_census;
+  A.np = s->np;
+  A.nclusters = s->nb.nclusters;
+  A.cutoff2 = s->nb.cutoff2;
+  A.alpha = (float)s->alpha;
+  A.beta = (float)(2.0 * s->alpha / std::sqrt(M_PI));
+  A.exp_scale = (float)(-s->alpha * s->alpha / std::log(2.0));
+  if (energy || census)
+    k_nb_pairs<true><<<B.segment_capacity, 64, 0, st>>>(A);
+  else
+    k_nb_pairs<false><<<B.segment_capacity, 64, 0, st>>>(A);
+  MD_HIP(hipGetLastError());
+}
+
+int64_t md_nb_entries(md_system *s) {
+  int t[2];
+  MD_HIP(hipMemcpy(t, nbuf(s).totals, sizeof t, hipMemcpyDeviceToHost));
+  return t[0];
+}
+
+int64_t md_nb_capacity(md_system *s) { return nbuf(s).entry_capacity; }

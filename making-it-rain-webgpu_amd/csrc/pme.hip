@@ -1,0 +1,508 @@
+// pme.hip -- smooth particle-mesh Ewald for gfx950.
+//
+// Replaces the upstream WGSL `spread_charge` (src/gpu/kernels/pme.wgsl:123-154),
+// `load_grid` (pme-grid.wgsl:13-19), `fft_stage` (fft.wgsl:59-107), `convolve`
+// (pme-convolve.wgsl:33-91) and `interpolate_force` (pme-interpolate.wgsl
+// :104-155).  The mathematics is src/reference/forces/pme.ts:94-263: order-5
+// cardinal B-splines at fractional coordinates (grid points floor(u)-4 ..
+// floor(u)), E = (2 pi K / V) sum_{m != 0} exp(-k^2/4 alpha^2) / (k^2 |b(m)|^2)
+// |Q(m)|^2 with k = 2 pi m B^-T, the grid multiplied by twice the factor, and
+// forces -q B^-1 (n_x dtheta_x theta_y theta_z, ...) . phi.
+//
+// gfx950 design: the charge grid is int64 Q32.32 (125 atomic adds per charged
+// atom, order independent); the transform is real-to-complex, so only
+// nz/2+1 planes of k-space are stored and transformed; each 1D transform runs
+// whole inside LDS (a batch of lines per workgroup, mixed-radix Stockham
+// stages, twiddles from an f64-computed table); the x-axis pass does forward
+// transform, convolution, energy and inverse transform in one kernel; the
+// z pass that reads the integer grid also clears it for the next step.
+#include <cmath>
+#include <vector>
+
+#include "engine.hpp"
+
+#define PME_MAX_STAGES 12
+#define PME_BATCH 16
+
+struct md_fft_plan {
+  int n;
+  int nstages;
+  int radix[PME_MAX_STAGES];
+  int tw;  // offset of this length's twiddle table
+};
+
+struct md_pme_args {
+  int nx, ny, nz, nzc;
+  md_fft_plan px, py, pz;
+  const float2 *tw;  // exp(-2 pi i t / n), per length
+  const float *mod_x, *mod_y, *mod_z;
+  double inv[9];
+  float invf[9];
+  float prefactor;   // K 2 pi / V
+  float exp_scale;   // -1/(4 alpha^2)
+};
+
+// ---- B-splines (src/reference/forces/bspline.ts:55-91, order 5) ------------
+
+__device__ __forceinline__ void md_bspline5(float w, float *th, float *dth) {
+  float l0 = 1.0f - w, l1 = w, l2, l3;
+  // order 3
+  l2 = 0.5f * w * l1;
+  l1 = 0.5f * ((w + 1.0f) * l0 + (2.0f - w) * l1);
+  l0 = 0.5f * (1.0f - w) * l0;
+  // order 4
+  const float third = 1.0f / 3.0f;
+  l3 = third * w * l2;
+  l2 = third * ((w + 1.0f) * l1 + (3.0f - w) * l2);
+  l1 = third * ((w + 2.0f) * l0 + (2.0f - w) * l1);
+  l0 = third * (1.0f - w) * l0;
+  if (dth) {
+    dth[0] = -l0;
+    dth[1] = l0 - l1;
+    dth[2] = l1 - l2;
+    dth[3] = l2 - l3;
+    dth[4] = l3;
+  }
+  // order 5
+  th[4] = 0.25f * w * l3;
+  th[3] = 0.25f * ((w + 1.0f) * l2 + (4.0f - w) * l3);
+  th[2] = 0.25f * ((w + 2.0f) * l1 + (3.0f - w) * l2);
+  th[1] = 0.25f * ((w + 3.0f) * l0 + (2.0f - w) * l1);
+  th[0] = 0.25f * (1.0f - w) * l0;
+}
+
+// fractional position on the mesh, from the exact fixed-point coordinate
+__device__ __forceinline__ void md_mesh_coord(const int64_t *pos, int np, int i, const double *M, int nx, int ny,
+                                              int nz, int *base, float *w) {
+  const double s = 2.3283064365386963e-10;
+  double rx = (double)pos[i] * s, ry = (double)pos[np + i] * s, rz = (double)pos[2 * np + i] * s;
+  double f[3] = {rx * M[0] + ry * M[3] + rz * M[6], ry * M[4] + rz * M[7], rz * M[8]};
+  int n[3] = {nx, ny, nz};
+  for (int d = 0; d < 3; ++d) {
+    double fr = f[d] - floor(f[d]);
+    double u = fr * n[d];
+    double cell = floor(u);
+    w[d] = (float)(u - cell);
+    base[d] = (int)cell - (MD_PME_ORDER - 1);
+  }
+}
+
+__device__ __forceinline__ int md_wrap(int i, int n) {
+  i %= n;
+  return i < 0 ? i + n : i;
+}
+
+// one thread per (atom, x-plane of the stencil): 25 integer atomics each
+__global__ void k_pme_spread(int n, int np, const int64_t *__restrict__ pos, const float4 *__restrict__ param,
+                             md_pme_args P, int64_t *grid) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = t / MD_PME_ORDER, a = t - i * MD_PME_ORDER;
+  if (i >= n) return;
+  float q = param[i].w;
+  if (q == 0.0f) return;
+  int base[3];
+  float w[3];
+  md_mesh_coord(pos, np, i, P.inv, P.nx, P.ny, P.nz, base, w);
+  float tx[5], ty[5], tz[5];
+  md_bspline5(w[0], tx, nullptr);
+  md_bspline5(w[1], ty, nullptr);
+  md_bspline5(w[2], tz, nullptr);
+  int gx = md_wrap(base[0] + a, P.nx);
+  float wx = q * tx[a];
+  for (int b = 0; b < 5; ++b) {
+    int gy = md_wrap(base[1] + b, P.ny);
+    float wxy = wx * ty[b];
+    int64_t *row = grid + ((size_t)gx * P.ny + gy) * P.nz;
+    for (int c = 0; c < 5; ++c) {
+      int gz = md_wrap(base[2] + c, P.nz);
+      md_atomic_add_i64(&row[gz], md_to_fixed(wxy * tz[c]));
+    }
+  }
+}
+
+// ---- in-LDS mixed-radix Stockham transform ---------------------------------
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// `nlines` contiguous lines of length P.n in `a` (pitch P.n), transformed with
+// sign -1 (forward) or +1 (inverse); returns the buffer holding the result.
+__device__ float2 *md_lds_fft(float2 *a, float2 *b, int nlines, const md_fft_plan &P, const float2 *tw, float sign) {
+  int n = P.n;
+  int ns = 1;
+  float2 *in = a, *out = b;
+  for (int s = 0; s < P.nstages; ++s) {
+    int r = P.radix[s];
+    int m = n / r;
+    int twstride = n / (ns * r);
+    int dftstride = n / r;
+    for (int idx = threadIdx.x; idx < nlines * m; idx += blockDim.x) {
+      int line = idx / m, j = idx - line * m;
+      int k = j % ns;
+      float2 v[16];
+      const float2 *src = in + line * n;
+      for (int q = 0; q < r; ++q) {
+        float2 x = src[j + q * m];
+        if (q > 0 && k > 0) {
+          float2 w = tw[P.tw + (k * q * twstride) % n];
+          w.y *= -sign;  // table holds exp(-2 pi i t/n); inverse conjugates
+          x = cmul(x, w);
+        }
+        v[q] = x;
+      }
+      float2 *dst = out + line * n + (j / ns) * ns * r + k;
+      if (r == 2) {
+        dst[0] = make_float2(v[0].x + v[1].x, v[0].y + v[1].y);
+        dst[ns] = make_float2(v[0].x - v[1].x, v[0].y - v[1].y);
+      } else if (r == 4) {
+        float2 s0 = make_float2(v[0].x + v[2].x, v[0].y + v[2].y);
+        float2 s1 = make_float2(v[0].x - v[2].x, v[0].y - v[2].y);
+        float2 s2 = make_float2(v[1].x + v[3].x, v[1].y + v[3].y);
+        float2 s3 = make_float2(v[1].x - v[3].x, v[1].y - v[3].y);
+        // multiply s3 by sign*i: forward (-i), inverse (+i)
+        float2 t3 = make_float2(-sign * s3.y, sign * s3.x);
+        dst[0] = make_float2(s0.x + s2.x, s0.y + s2.y);
+        dst[ns] = make_float2(s1.x + t3.x, s1.y + t3.y);
+        dst[2 * ns] = make_float2(s0.x - s2.x, s0.y - s2.y);
+        dst[3 * ns] = make_float2(s1.x - t3.x, s1.y - t3.y);
+      } else {
+        for (int qo = 0; qo < r; ++qo) {
+          float2 acc = v[0];
+          for (int q = 1; q < r; ++q) {
+            float2 w = tw[P.tw + ((qo * q) % r) * dftstride];
+            w.y *= -sign;
+            float2 p = cmul(v[q], w);
+            acc.x += p.x;
+            acc.y += p.y;
+          }
+          dst[qo * ns] = acc;
+        }
+      }
+    }
+    __syncthreads();
+    float2 *t = in;
+    in = out;
+    out = t;
+    ns *= r;
+  }
+  return in;
+}
+
+// z pass, forward: int64 rows -> f32 -> complex FFT -> first nz/2+1 values.
+// Clears the integer grid behind it.  Block = PME_BATCH rows.
+__global__ void __launch_bounds__(256) k_pme_z_forward(md_pme_args P, int64_t *grid, float2 *cgrid) {
+  extern __shared__ float2 lds[];
+  int rows = P.nx * P.ny;
+  int r0 = blockIdx.x * PME_BATCH;
+  int nl = min(PME_BATCH, rows - r0);
+  float2 *a = lds, *b = lds + PME_BATCH * P.nz;
+  for (int idx = threadIdx.x; idx < nl * P.nz; idx += blockDim.x) {
+    int l = idx / P.nz, z = idx - l * P.nz;
+    size_t at = (size_t)(r0 + l) * P.nz + z;
+    int64_t v = grid[at];
+    grid[at] = 0;
+    a[l * P.nz + z] = make_float2((float)((double)v * 2.3283064365386963e-10), 0.0f);
+  }
+  __syncthreads();
+  float2 *res = md_lds_fft(a, b, nl, P.pz, P.tw, -1.0f);
+  for (int idx = threadIdx.x; idx < nl * P.nzc; idx += blockDim.x) {
+    int l = idx / P.nzc, z = idx - l * P.nzc;
+    cgrid[(size_t)(r0 + l) * P.nzc + z] = res[l * P.nz + z];
+  }
+}
+
+// y pass: lines along y at fixed (x, kz); a block takes PME_BATCH
+// consecutive kz at one x, so global reads are contiguous runs.
+__global__ void __launch_bounds__(256) k_pme_y(md_pme_args P, float2 *cgrid, float sign) {
+  extern __shared__ float2 lds[];
+  int kzb = (P.nzc + PME_BATCH - 1) / PME_BATCH;
+  int x = blockIdx.x / kzb;
+  int z0 = (blockIdx.x - x * kzb) * PME_BATCH;
+  int nl = min(PME_BATCH, P.nzc - z0);
+  float2 *a = lds, *b = lds + PME_BATCH * P.ny;
+  for (int idx = threadIdx.x; idx < nl * P.ny; idx += blockDim.x) {
+    int y = idx / nl, l = idx - y * nl;
+    a[l * P.ny + y] = cgrid[((size_t)x * P.ny + y) * P.nzc + z0 + l];
+  }
+  __syncthreads();
+  float2 *res = md_lds_fft(a, b, nl, P.py, P.tw, sign);
+  for (int idx = threadIdx.x; idx < nl * P.ny; idx += blockDim.x) {
+    int y = idx / nl, l = idx - y * nl;
+    cgrid[((size_t)x * P.ny + y) * P.nzc + z0 + l] = res[l * P.ny + y];
+  }
+}
+
+// x pass with the convolution: forward transform along x, multiply by the
+// Ewald influence function (accumulating the energy), inverse transform.
+template <bool ENERGY>
+__global__ void __launch_bounds__(256) k_pme_x_convolve(md_pme_args P, float2 *cgrid, int64_t *energy) {
+  extern __shared__ float2 lds[];
+  int kzb = (P.nzc + PME_BATCH - 1) / PME_BATCH;
+  int y = blockIdx.x / kzb;
+  int z0 = (blockIdx.x - y * kzb) * PME_BATCH;
+  int nl = min(PME_BATCH, P.nzc - z0);
+  float2 *a = lds, *b = lds + PME_BATCH * P.nx;
+  size_t xstride = (size_t)P.ny * P.nzc;
+  for (int idx = threadIdx.x; idx < nl * P.nx; idx += blockDim.x) {
+    int x = idx / nl, l = idx - x * nl;
+    a[l * P.nx + x] = cgrid[x * xstride + (size_t)y * P.nzc + z0 + l];
+  }
+  __syncthreads();
+  float2 *res = md_lds_fft(a, b, nl, P.px, P.tw, -1.0f);
+  float e = 0.0f;
+  int mb = y <= P.ny / 2 ? y : y - P.ny;
+  const float two_pi = 6.283185307179586f;
+  for (int idx = threadIdx.x; idx < nl * P.nx; idx += blockDim.x) {
+    int l = idx / P.nx, x = idx - l * P.nx;
+    int kz = z0 + l;
+    int ma = x <= P.nx / 2 ? x : x - P.nx;
+    int mc = kz;
+    float2 v = res[l * P.nx + x];
+    float2 outv = make_float2(0.0f, 0.0f);
+    if (!(x == 0 && y == 0 && kz == 0)) {
+      float kx = two_pi * (ma * P.invf[0]);
+      float ky = two_pi * (ma * P.invf[3] + mb * P.invf[4]);
+      float kzv = two_pi * (ma * P.invf[6] + mb * P.invf[7] + mc * P.invf[8]);
+      float k2 = kx * kx + ky * ky + kzv * kzv;
+      float mod = P.mod_x[x] * P.mod_y[y] * P.mod_z[kz];
+      float factor = P.prefactor * __expf(k2 * P.exp_scale) / (k2 * mod);
+      if (ENERGY) {
+        // the half grid stands for both k and -k except on the kz = 0 and
+        // kz = nz/2 planes, which are their own partners
+        float weight = (kz == 0 || (2 * kz == P.nz)) ? 1.0f : 2.0f;
+        e += weight * factor * (v.x * v.x + v.y * v.y);
+      }
+      outv = make_float2(2.0f * factor * v.x, 2.0f * factor * v.y);
+    }
+    res[l * P.nx + x] = outv;
+  }
+  __syncthreads();
+  float2 *other = res == a ? b : a;
+  float2 *res2 = md_lds_fft(res, other, nl, P.px, P.tw, 1.0f);
+  for (int idx = threadIdx.x; idx < nl * P.nx; idx += blockDim.x) {
+    int x = idx / nl, l = idx - x * nl;
+    cgrid[x * xstride + (size_t)y * P.nzc + z0 + l] = res2[l * P.nx + x];
+  }
+  if (ENERGY) {
+    int64_t f = md_wave_sum_i64(md_to_fixed(e));
+    if ((threadIdx.x & 63) == 0) md_atomic_add_i64(energy + MD_E_RECIPROCAL, f);
+  }
+}
+
+// z pass, inverse: Hermitian half line -> full line -> inverse FFT -> real.
+__global__ void __launch_bounds__(256) k_pme_z_inverse(md_pme_args P, const float2 *cgrid, float *phi) {
+  extern __shared__ float2 lds[];
+  int rows = P.nx * P.ny;
+  int r0 = blockIdx.x * PME_BATCH;
+  int nl = min(PME_BATCH, rows - r0);
+  float2 *a = lds, *b = lds + PME_BATCH * P.nz;
+  for (int idx = threadIdx.x; idx < nl * P.nz; idx += blockDim.x) {
+    int l = idx / P.nz, z = idx - l * P.nz;
+    float2 v;
+    if (z < P.nzc) {
+      v = cgrid[(size_t)(r0 + l) * P.nzc + z];
+    } else {
+      // x and y are back in real space, so each row is the z transform of a
+      // real line: g(x, y, -kz) = conj(g(x, y, kz)), same row
+      float2 w = cgrid[(size_t)(r0 + l) * P.nzc + (P.nz - z)];
+      v = make_float2(w.x, -w.y);
+    }
+    a[l * P.nz + z] = v;
+  }
+  __syncthreads();
+  float2 *res = md_lds_fft(a, b, nl, P.pz, P.tw, 1.0f);
+  for (int idx = threadIdx.x; idx < nl * P.nz; idx += blockDim.x) {
+    int l = idx / P.nz, z = idx - l * P.nz;
+    phi[(size_t)(r0 + l) * P.nz + z] = res[l * P.nz + z].x;
+  }
+}
+
+// interpolate dE/dQ back onto the atoms: one thread per atom
+__global__ void k_pme_gather(int n, int np, const int64_t *__restrict__ pos, const float4 *__restrict__ param,
+                             md_pme_args P, const float *__restrict__ phi, int64_t *force) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float q = param[i].w;
+  if (q == 0.0f) return;
+  int base[3];
+  float w[3];
+  md_mesh_coord(pos, np, i, P.inv, P.nx, P.ny, P.nz, base, w);
+  float tx[5], ty[5], tz[5], dx[5], dy[5], dz[5];
+  md_bspline5(w[0], tx, dx);
+  md_bspline5(w[1], ty, dy);
+  md_bspline5(w[2], tz, dz);
+  int gz[5];
+  for (int c = 0; c < 5; ++c) gz[c] = md_wrap(base[2] + c, P.nz);
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  for (int a = 0; a < 5; ++a) {
+    int gx = md_wrap(base[0] + a, P.nx);
+    for (int b = 0; b < 5; ++b) {
+      int gy = md_wrap(base[1] + b, P.ny);
+      const float *row = phi + ((size_t)gx * P.ny + gy) * P.nz;
+      float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+      for (int c = 0; c < 5; ++c) {
+        float p = row[gz[c]];
+        sz += p * dz[c];
+        sx += p * tz[c];
+      }
+      fx += dx[a] * ty[b] * sx;
+      fy += tx[a] * dy[b] * sx;
+      fz += tx[a] * ty[b] * sz;
+    }
+  }
+  // d/df chain rule back to Cartesian (src/reference/forces/pme.ts:246-260)
+  float dfx = -q * fx * P.nx, dfy = -q * fy * P.ny, dfz = -q * fz * P.nz;
+  const float *M = P.invf;
+  float Fx = dfx * M[0] + dfy * M[1] + dfz * M[2];
+  float Fy = dfx * M[3] + dfy * M[4] + dfz * M[5];
+  float Fz = dfx * M[6] + dfy * M[7] + dfz * M[8];
+  md_atomic_add_i64(&force[i], md_to_fixed(Fx));
+  md_atomic_add_i64(&force[np + i], md_to_fixed(Fy));
+  md_atomic_add_i64(&force[2 * np + i], md_to_fixed(Fz));
+}
+
+// ---- host -------------------------------------------------------------------
+
+struct md_pme_host {
+  md_fft_plan px, py, pz;
+  float *mod = nullptr;  // x | y | z
+  float2 *tw = nullptr;
+};
+
+static md_fft_plan make_plan(int n, int tw_offset) {
+  md_fft_plan p{};
+  p.n = n;
+  p.tw = tw_offset;
+  int r = n;
+  // radix 4 first, then 2, 3, 5, 7, then any remaining prime (direct DFT)
+  while (r % 4 == 0) {
+    if (p.nstages >= PME_MAX_STAGES) break;
+    p.radix[p.nstages++] = 4;
+    r /= 4;
+  }
+  for (int f : {2, 3, 5, 7, 11, 13})
+    while (r % f == 0) {
+      if (p.nstages >= PME_MAX_STAGES) throw md_error(MD_ERR_NOT_SMOOTH, "mesh size has too many factors");
+      p.radix[p.nstages++] = f;
+      r /= f;
+    }
+  if (r != 1)
+    throw md_error(MD_ERR_NOT_SMOOTH, std::to_string(n) + " has a prime factor above 13 (" + std::to_string(r) +
+                                          " is left); PME mesh sizes must be 13-smooth");
+  return p;
+}
+
+// |b(m)|^2 of the order-5 spline with the 1e-7 neighbour guard
+// (src/reference/forces/bspline.ts:102-132), as its reciprocal's factor
+static std::vector<double> moduli(int n) {
+  const double knots[5] = {1.0 / 24, 11.0 / 24, 11.0 / 24, 1.0 / 24, 0.0};
+  std::vector<double> m(n);
+  for (int k = 0; k < n; ++k) {
+    double re = 0, im = 0;
+    for (int j = 0; j < 5 && j < n; ++j) {
+      double a = 2.0 * M_PI * (double)k * j / n;
+      re += knots[j] * std::cos(a);
+      im += knots[j] * std::sin(a);
+    }
+    m[k] = re * re + im * im;
+  }
+  for (int k = 0; k < n; ++k)
+    if (m[k] < 1e-7) m[k] = 0.5 * (m[(k - 1 + n) % n] + m[(k + 1) % n]);
+  return m;
+}
+
+static md_pme_host &pme_host(md_system *s) { return *reinterpret_cast<md_pme_host *>(s->pme_ext); }
+
+void md_pme_setup(md_system *s) {
+  s->pme_ext = new md_pme_host();
+  auto &H = pme_host(s);
+  int nx = s->grid[0], ny = s->grid[1], nz = s->grid[2];
+  s->pme_nx = nx;
+  s->pme_ny = ny;
+  s->pme_nz = nz;
+  s->pme_nzc = nz / 2 + 1;
+  int lens[3] = {nx, ny, nz};
+  std::vector<float2> tw;
+  int off[3];
+  for (int d = 0; d < 3; ++d) {
+    off[d] = (int)tw.size();
+    for (int t = 0; t < lens[d]; ++t) {
+      double a = -2.0 * M_PI * t / lens[d];
+      tw.push_back(make_float2((float)std::cos(a), (float)std::sin(a)));
+    }
+  }
+  H.px = make_plan(nx, off[0]);
+  H.py = make_plan(ny, off[1]);
+  H.pz = make_plan(nz, off[2]);
+  std::vector<float> mod;
+  for (int d = 0; d < 3; ++d)
+    for (double v : moduli(lens[d])) mod.push_back((float)v);
+  H.mod = md_alloc<float>(s, mod.size());
+  H.tw = md_alloc<float2>(s, tw.size());
+  MD_HIP(hipMemcpyAsync(H.mod, mod.data(), sizeof(float) * mod.size(), hipMemcpyHostToDevice, s->stream));
+  MD_HIP(hipMemcpyAsync(H.tw, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice, s->stream));
+  size_t g = (size_t)nx * ny * nz;
+  s->pme_grid = md_alloc<int64_t>(s, g);
+  s->pme_cgrid = md_alloc<float2>(s, (size_t)nx * ny * s->pme_nzc);
+  s->pme_phi = md_alloc<float>(s, g);
+  MD_HIP(hipStreamSynchronize(s->stream));
+  int maxn = std::max(nx, std::max(ny, nz));
+  size_t lds = sizeof(float2) * 2 * PME_BATCH * maxn;
+  if (lds > 64 * 1024) {
+    for (auto *k : {(const void *)k_pme_z_forward, (const void *)k_pme_y, (const void *)k_pme_x_convolve<true>,
+                    (const void *)k_pme_x_convolve<false>, (const void *)k_pme_z_inverse})
+      MD_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  }
+}
+
+void md_pme_free(md_system *s) {
+  delete reinterpret_cast<md_pme_host *>(s->pme_ext);
+  s->pme_ext = nullptr;
+}
+
+static md_pme_args pme_args(md_system *s) {
+  auto &H = pme_host(s);
+  md_pme_args P;
+  P.nx = s->pme_nx;
+  P.ny = s->pme_ny;
+  P.nz = s->pme_nz;
+  P.nzc = s->pme_nzc;
+  P.px = H.px;
+  P.py = H.py;
+  P.pz = H.pz;
+  P.tw = H.tw;
+  P.mod_x = H.mod;
+  P.mod_y = H.mod + P.nx;
+  P.mod_z = H.mod + P.nx + P.ny;
+  for (int k = 0; k < 9; ++k) {
+    P.inv[k] = s->dbox.invd[k];
+    P.invf[k] = s->dbox.inv[k];
+  }
+  double volume = s->box[0] * s->box[4] * s->box[8];
+  P.prefactor = (float)(MD_ONE_4PI_EPS0 * 2.0 * M_PI / volume);
+  P.exp_scale = (float)(-1.0 / (4.0 * s->alpha * s->alpha));
+  return P;
+}
+
+void md_pme(md_system *s, bool energy) {
+  md_pme_args P = pme_args(s);
+  hipStream_t st = s->stream;
+  int n = s->n;
+  k_pme_spread<<<(n * MD_PME_ORDER + 255) / 256, 256, 0, st>>>(n, s->np, s->pos, s->atom_param, P, s->pme_grid);
+  int rows = P.nx * P.ny;
+  size_t lz = sizeof(float2) * 2 * PME_BATCH * P.nz;
+  size_t ly = sizeof(float2) * 2 * PME_BATCH * P.ny;
+  size_t lx = sizeof(float2) * 2 * PME_BATCH * P.nx;
+  int kzb = (P.nzc + PME_BATCH - 1) / PME_BATCH;
+  k_pme_z_forward<<<(rows + PME_BATCH - 1) / PME_BATCH, 256, lz, st>>>(P, s->pme_grid, s->pme_cgrid);
+  k_pme_y<<<P.nx * kzb, 256, ly, st>>>(P, s->pme_cgrid, -1.0f);
+  if (energy)
+    k_pme_x_convolve<true><<<P.ny * kzb, 256, lx, st>>>(P, s->pme_cgrid, s->energy);
+  else
+    k_pme_x_convolve<false><<<P.ny * kzb, 256, lx, st>>>(P, s->pme_cgrid, s->energy);
+  k_pme_y<<<P.nx * kzb, 256, ly, st>>>(P, s->pme_cgrid, 1.0f);
+  k_pme_z_inverse<<<(rows + PME_BATCH - 1) / PME_BATCH, 256, lz, st>>>(P, s->pme_cgrid, s->pme_phi);
+  k_pme_gather<<<(n + 255) / 256, 256, 0, st>>>(n, s->np, s->pos, s->atom_param, P, s->pme_phi, s->force);
+  MD_HIP(hipGetLastError());
+}

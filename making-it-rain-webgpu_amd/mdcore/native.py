@@ -1,0 +1,333 @@
+"""ctypes binding of ``libmdcore.so`` (the C ABI in ``include/mdcore.h``).
+
+This is the reference-side binding a host would add in place of the WebGPU
+device: it marshals an ``OpenMMSystem`` into ``md_system_desc``, owns the
+opaque ``md_system*`` handle, and maps error codes onto the named errors the
+upstream host throws (``UnsupportedForce``, ``CutoffTooLargeForBox``,
+``ClusterTooLarge``, ``RangeError`` for fixed-point overflow...).  There is no
+fallback: if the library is missing or no GPU is visible, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmdcore.so")
+
+MD_GROUP_NAMES = ("HarmonicBondForce", "HarmonicAngleForce", "PeriodicTorsionForce",
+                  "NonbondedForce[direct]", "NonbondedForce[reciprocal]", "PositionRestraint")
+
+# md_err codes
+MD_OK = 0
+ERRORS = {
+    1: "UnsupportedForce", 2: "CutoffTooLargeForBox", 3: "ClusterTooLarge",
+    4: "FixedPointOverflow", 5: "HipError", 6: "OutOfMemory", 7: "InvalidArgument",
+    8: "ListOverflow", 9: "NotSmooth", 10: "ConstraintsDiverged", 11: "FaultInjected",
+    12: "NoDevice",
+}
+
+
+class MdError(RuntimeError):
+    """A non-zero ``md_err``; ``name`` is the upstream error class it maps to."""
+
+    def __init__(self, code: int, message: str):
+        self.code = code
+        self.name = ERRORS.get(code, f"md_err {code}")
+        super().__init__(f"{self.name}: {message}")
+
+
+class CutoffTooLargeForBox(MdError):
+    pass
+
+
+class ClusterTooLarge(MdError):
+    pass
+
+
+class FixedPointOverflow(MdError, ArithmeticError):
+    pass
+
+
+class FaultInjected(MdError):
+    pass
+
+
+class NoDevice(MdError):
+    pass
+
+
+_SPECIFIC = {2: CutoffTooLargeForBox, 3: ClusterTooLarge, 4: FixedPointOverflow, 11: FaultInjected,
+             12: NoDevice}
+
+_ip = C.POINTER(C.c_int32)
+_dp = C.POINTER(C.c_double)
+
+
+class SystemDesc(C.Structure):
+    _fields_ = [
+        ("n_particles", C.c_int32), ("masses", _dp), ("box", C.c_double * 9),
+        ("n_constraints", C.c_int32), ("constraint_p1", _ip), ("constraint_p2", _ip),
+        ("constraint_distance", _dp),
+        ("n_bonds", C.c_int32), ("bond_p1", _ip), ("bond_p2", _ip), ("bond_length", _dp), ("bond_k", _dp),
+        ("n_angles", C.c_int32), ("angle_p1", _ip), ("angle_p2", _ip), ("angle_p3", _ip),
+        ("angle_theta", _dp), ("angle_k", _dp),
+        ("n_torsions", C.c_int32), ("torsion_p1", _ip), ("torsion_p2", _ip), ("torsion_p3", _ip),
+        ("torsion_p4", _ip), ("torsion_periodicity", _ip), ("torsion_phase", _dp), ("torsion_k", _dp),
+        ("has_nonbonded", C.c_int32), ("cutoff", C.c_double), ("ewald_tolerance", C.c_double),
+        ("alpha", C.c_double), ("grid", C.c_int32 * 3), ("use_dispersion_correction", C.c_int32),
+        ("charge", _dp), ("sigma", _dp), ("epsilon", _dp),
+        ("n_exceptions", C.c_int32), ("exception_p1", _ip), ("exception_p2", _ip),
+        ("exception_chargeprod", _dp), ("exception_sigma", _dp), ("exception_epsilon", _dp),
+        ("n_restraints", C.c_int32), ("restraint_particle", _ip), ("restraint_k", _dp),
+        ("restraint_x0", _dp), ("restraint_y0", _dp), ("restraint_z0", _dp),
+        ("cmm_frequency", C.c_int32),
+        ("has_barostat", C.c_int32), ("barostat_pressure", C.c_double),
+        ("barostat_temperature", C.c_double), ("barostat_frequency", C.c_int32),
+        ("barostat_seed", C.c_uint32),
+    ]
+
+
+class RunOptionsC(C.Structure):
+    _fields_ = [("time_step", C.c_double), ("temperature", C.c_double), ("friction", C.c_double),
+                ("constraint_tolerance", C.c_double), ("seed", C.c_uint32), ("integrator", C.c_int32),
+                ("verlet_skin", C.c_double), ("mesh", C.c_int32 * 3), ("device", C.c_int32)]
+
+
+class ListFlags(C.Structure):
+    _fields_ = [("rebuilds", C.c_int64), ("tiles", C.c_int64), ("tile_capacity", C.c_int64),
+                ("worst_displacement", C.c_double), ("overflowed", C.c_int32)]
+
+
+_lib = None
+
+
+def library():
+    """The loaded ``libmdcore.so``; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() or "
+                              "`make -C making-it-rain-webgpu_amd`")
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        L.md_last_error.restype = C.c_char_p
+        L.md_version.restype = C.c_char_p
+        L.md_system_create.argtypes = [P(SystemDesc), P(RunOptionsC), P(C.c_void_p)]
+        L.md_system_destroy.argtypes = [C.c_void_p]
+        L.md_system_destroy.restype = None
+        for name in ("md_set_positions", "md_set_velocities", "md_get_positions", "md_get_velocities",
+                     "md_set_box", "md_get_box"):
+            getattr(L, name).argtypes = [C.c_void_p, _dp]
+        L.md_evaluate.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.md_run.argtypes = [C.c_void_p, C.c_int64]
+        L.md_get_step.argtypes = [C.c_void_p, P(C.c_int64)]
+        L.md_set_step.argtypes = [C.c_void_p, C.c_int64]
+        L.md_list_flags_get.argtypes = [C.c_void_p, P(ListFlags)]
+        L.md_pair_census.argtypes = [C.c_void_p, P(C.c_int64), P(C.c_int64)]
+        L.md_probe_fixed_point.argtypes = [P(C.c_float), C.c_int32, P(C.c_int64), P(C.c_float)]
+        L.md_probe_thermostat.argtypes = [C.c_uint32, C.c_uint32, C.c_int32, P(C.c_uint32), P(C.c_float)]
+        L.md_inject_fault.argtypes = [C.c_void_p, C.c_int32]
+        _lib = L
+    return _lib
+
+
+def check(code: int) -> None:
+    if code != MD_OK:
+        msg = library().md_last_error().decode(errors="replace")
+        raise _SPECIFIC.get(code, MdError)(code, msg)
+
+
+def _d(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a: np.ndarray):
+    return a.ctypes.data_as(_ip)
+
+
+def describe(system, box: Optional[Sequence] = None):
+    """Pack an OpenMMSystem into ``md_system_desc``.  Returns (desc, keepalive)."""
+    keep = []
+
+    def f64(x):
+        a = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+        keep.append(a)
+        return _d(a)
+
+    def i32(x):
+        a = np.ascontiguousarray(np.asarray(x, dtype=np.int32))
+        keep.append(a)
+        return _i(a)
+
+    d = SystemDesc()
+    d.n_particles = system.particleCount
+    d.masses = f64(system.masses)
+    b = np.asarray(box if box is not None else system.boxVectors, dtype=np.float64).reshape(9)
+    for k in range(9):
+        d.box[k] = b[k]
+    d.n_constraints = system.constraintCount
+    if d.n_constraints:
+        d.constraint_p1, d.constraint_p2 = i32(system.constraintP1), i32(system.constraintP2)
+        d.constraint_distance = f64(system.constraintDistance)
+    for f in system.forces:
+        t = f.type
+        if t == "HarmonicBondForce" and f.count:
+            d.n_bonds, d.bond_p1, d.bond_p2 = f.count, i32(f.p1), i32(f.p2)
+            d.bond_length, d.bond_k = f64(f.length), f64(f.k)
+        elif t == "HarmonicAngleForce" and f.count:
+            d.n_angles, d.angle_p1, d.angle_p2, d.angle_p3 = f.count, i32(f.p1), i32(f.p2), i32(f.p3)
+            d.angle_theta, d.angle_k = f64(f.angle), f64(f.k)
+        elif t == "PeriodicTorsionForce" and f.count:
+            d.n_torsions = f.count
+            d.torsion_p1, d.torsion_p2, d.torsion_p3, d.torsion_p4 = (i32(f.p1), i32(f.p2), i32(f.p3),
+                                                                      i32(f.p4))
+            d.torsion_periodicity, d.torsion_phase, d.torsion_k = (i32(f.periodicity), f64(f.phase),
+                                                                   f64(f.k))
+        elif t == "NonbondedForce":
+            if f.method != 4:
+                raise MdError(1, f"NonbondedForce method {f.method}: this engine implements PME (4)")
+            d.has_nonbonded = 1
+            d.cutoff, d.ewald_tolerance, d.alpha = f.cutoff, f.ewaldTolerance, f.alpha
+            for k in range(3):
+                d.grid[k] = int(f.grid[k])
+            d.use_dispersion_correction = int(bool(f.useDispersionCorrection))
+            d.charge, d.sigma, d.epsilon = f64(f.charge), f64(f.sigma), f64(f.epsilon)
+            d.n_exceptions = f.exceptionCount
+            if f.exceptionCount:
+                d.exception_p1, d.exception_p2 = i32(f.exceptionP1), i32(f.exceptionP2)
+                d.exception_chargeprod = f64(f.exceptionChargeProd)
+                d.exception_sigma, d.exception_epsilon = f64(f.exceptionSigma), f64(f.exceptionEpsilon)
+        elif t == "PositionRestraint" and f.count:
+            d.n_restraints, d.restraint_particle, d.restraint_k = f.count, i32(f.particle), f64(f.k)
+            d.restraint_x0, d.restraint_y0, d.restraint_z0 = f64(f.x0), f64(f.y0), f64(f.z0)
+        elif t == "CMMotionRemover":
+            d.cmm_frequency = int(f.frequency)
+        elif t == "MonteCarloBarostat":
+            d.has_barostat = 1
+            d.barostat_pressure, d.barostat_temperature = f.defaultPressure, f.defaultTemperature
+            d.barostat_frequency, d.barostat_seed = int(f.frequency), int(f.randomSeed) & 0xFFFFFFFF
+    return d, keep
+
+
+class NativeSystem:
+    """An ``md_system*`` handle: one resident system on one GPU."""
+
+    def __init__(self, system, time_step=0.002, temperature=300.0, friction=1.0,
+                 constraint_tolerance=1e-6, seed=0, integrator="LangevinMiddle", verlet_skin=-1.0,
+                 mesh=(0, 0, 0), device=0, box=None):
+        L = library()
+        desc, keep = describe(system, box)
+        o = RunOptionsC()
+        o.time_step, o.temperature, o.friction = time_step, temperature, friction
+        o.constraint_tolerance = constraint_tolerance
+        o.seed = int(seed) & 0xFFFFFFFF
+        o.integrator = 0 if integrator == "LangevinMiddle" else 1
+        o.verlet_skin = verlet_skin
+        for k in range(3):
+            o.mesh[k] = int(mesh[k])
+        o.device = device
+        h = C.c_void_p()
+        check(L.md_system_create(C.byref(desc), C.byref(o), C.byref(h)))
+        self._h = h
+        self.n = system.particleCount
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            library().md_system_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise RuntimeError("the system has been destroyed")
+        return self._h
+
+    def set_positions(self, x):
+        a = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        check(library().md_set_positions(self.handle, _d(a)))
+
+    def set_velocities(self, v):
+        a = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        check(library().md_set_velocities(self.handle, _d(a)))
+
+    def positions(self) -> np.ndarray:
+        out = np.zeros(3 * self.n)
+        check(library().md_get_positions(self.handle, _d(out)))
+        return out
+
+    def velocities(self) -> np.ndarray:
+        out = np.zeros(3 * self.n)
+        check(library().md_get_velocities(self.handle, _d(out)))
+        return out
+
+    def set_box(self, box):
+        b = np.ascontiguousarray(np.asarray(box, dtype=np.float64).reshape(9))
+        check(library().md_set_box(self.handle, _d(b)))
+
+    def box(self) -> np.ndarray:
+        b = np.zeros(9)
+        check(library().md_get_box(self.handle, _d(b)))
+        return b.reshape(3, 3)
+
+    def evaluate(self, forces=True):
+        """(total, groups[6], forces[3N] or None)."""
+        f = np.zeros(3 * self.n) if forces else None
+        g = np.zeros(6)
+        t = C.c_double()
+        check(library().md_evaluate(self.handle, _d(f) if forces else None, _d(g), C.byref(t)))
+        return t.value, g, f
+
+    def run(self, steps: int):
+        check(library().md_run(self.handle, int(steps)))
+
+    @property
+    def step(self) -> int:
+        v = C.c_int64()
+        check(library().md_get_step(self.handle, C.byref(v)))
+        return v.value
+
+    @step.setter
+    def step(self, value: int):
+        check(library().md_set_step(self.handle, int(value)))
+
+    def list_flags(self) -> dict:
+        f = ListFlags()
+        check(library().md_list_flags_get(self.handle, C.byref(f)))
+        return {"rebuilds": f.rebuilds, "tiles": f.tiles, "tileCapacity": f.tile_capacity,
+                "worstDisplacement": f.worst_displacement, "overflowed": bool(f.overflowed)}
+
+    def pair_census(self):
+        p, t = C.c_int64(), C.c_int64()
+        check(library().md_pair_census(self.handle, C.byref(p), C.byref(t)))
+        return p.value, t.value
+
+    def inject_fault(self, after: int):
+        check(library().md_inject_fault(self.handle, int(after)))
+
+
+def probe_fixed_point(values):
+    v = np.ascontiguousarray(values, dtype=np.float32)
+    fixed = np.zeros(v.shape[0], dtype=np.int64)
+    back = np.zeros(v.shape[0], dtype=np.float32)
+    check(library().md_probe_fixed_point(v.ctypes.data_as(C.POINTER(C.c_float)), v.shape[0],
+                                         fixed.ctypes.data_as(C.POINTER(C.c_int64)),
+                                         back.ctypes.data_as(C.POINTER(C.c_float))))
+    return fixed, back
+
+
+def probe_thermostat(seed: int, step: int, n: int):
+    bits = np.zeros(4 * n, dtype=np.uint32)
+    normals = np.zeros(3 * n, dtype=np.float32)
+    check(library().md_probe_thermostat(seed, step, n, bits.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                        normals.ctypes.data_as(C.POINTER(C.c_float))))
+    return bits.reshape(n, 4), normals.reshape(n, 3)

@@ -1,0 +1,168 @@
+"""Parity of the HIP engine with the f64 oracle, through the C ABI, on the GPU.
+
+Gates (the upstream accuracy contract): forces within 2e-6 relative RMS of
+the f64 oracle (sqrt(sum (a-b)^2 / sum b^2) over 3N, src/gpu/verify.ts:145-155),
+per-force-group energies within 1e-5 relative, integer primitives bit-exact,
+and repeated runs bit-identical.  Mirrors test/nonbonded-reference.test.ts,
+test/pme-reference.test.ts and the browser-lane validators
+(src/gpu/validate.ts:80-117, validate-loop.ts:167-300) upstream.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import GOLDEN
+from mdcore import native, synthetic
+from mdcore.state import parseState
+from mdcore.system import parseSystem
+
+pytestmark = pytest.mark.gpu
+
+FORCE_RMS = 2e-6
+ENERGY_REL = 1e-5
+
+
+def rel_rms(a, b):
+    return float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2)))
+
+
+@pytest.fixture(scope="module")
+def fixture10():
+    with open(os.path.join(GOLDEN, "nonbonded-system.xml")) as f:
+        system = parseSystem(f.read())
+    with open(os.path.join(GOLDEN, "nonbonded-state.xml")) as f:
+        state = parseState(f.read())
+    with open(os.path.join(GOLDEN, "nonbonded-facts.json")) as f:
+        facts = json.load(f)
+    return system, state, facts
+
+
+@pytest.fixture(scope="module")
+def c0():
+    return synthetic.config(0)
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return synthetic.config(1)
+
+
+# ---- primitives -------------------------------------------------------------
+
+def test_fixed_point_conversion_is_bit_exact():
+    rng = np.random.default_rng(1)
+    vals = (rng.uniform(-1, 1, 20000) * 10.0 ** rng.uniform(-9, 6, 20000)).astype(np.float32)
+    vals = np.concatenate([vals, np.float32([0.0, -0.0, 2.5, -2.5, 1e-10, -1e-10, 2.0 ** -33])])
+    fixed, back = native.probe_fixed_point(vals)
+    want = np.array([O.to_fixed(float(v)) for v in vals], dtype=np.int64)
+    assert np.array_equal(fixed, want)
+    want_back = np.array([O.from_fixed_f32(int(f)) for f in want], dtype=np.float32)
+    assert np.array_equal(back, want_back)
+
+
+def test_thermostat_stream_matches_threefry():
+    seed, step, n = 0xC0FFEE, 12345, 4096
+    bits, normals = native.probe_thermostat(seed, step, n)
+    for p in range(0, n, 37):
+        a = O.threefry((2 * p, step), (seed, 0x9E3779B9))
+        b = O.threefry((2 * p + 1, step), (seed, 0x9E3779B9))
+        assert tuple(bits[p]) == a + b
+    L = O.lib()
+    want = np.array([[L.orc_thermostat_normal(seed, step, p, a) for a in range(3)] for p in range(n)])
+    # Box-Muller in f32 against f64: the uniforms are identical, the
+    # transcendentals differ by a few ulp
+    assert np.max(np.abs(normals - want)) < 2e-5
+
+
+# ---- the upstream fixture: OpenMM 8.4 numbers -----------------------------------
+
+def test_fixture_energies_match_openmm(fixture10):
+    system, state, facts = fixture10
+    s = native.NativeSystem(system, mesh=facts["PME"]["grid"])
+    s.set_positions(state.positions)
+    total, groups, forces = s.evaluate()
+    assert groups[3] == pytest.approx(facts["PME"]["direct_group"], rel=ENERGY_REL)
+    assert groups[4] == pytest.approx(facts["PME"]["reciprocal_group"], rel=ENERGY_REL)
+    want = np.asarray(facts["PME"]["forces"]).reshape(-1)
+    assert rel_rms(forces, want) < 1e-5  # against OpenMM's own mixed arithmetic
+    o = O.OracleSystem(system, grid=facts["PME"]["grid"])
+    _, og, of = o.potential(state.positions, system.boxVectors, grid=facts["PME"]["grid"])
+    assert rel_rms(forces, of) < FORCE_RMS
+    for k in (3, 4):
+        assert groups[k] == pytest.approx(og[k], rel=ENERGY_REL)
+    s.close()
+
+
+# ---- synthetic systems against the oracle ------------------------------------------
+
+def _compare(built, mesh=None):
+    system, state = built.system, built.state
+    o = O.OracleSystem(system)
+    grid = mesh or o.default_grid(system.boxVectors)
+    s = native.NativeSystem(system, mesh=grid)
+    s.set_positions(state.positions)
+    total, groups, forces = s.evaluate()
+    otot, og, of = o.potential(state.positions, system.boxVectors, grid=grid)
+    return s, (total, groups, forces), (otot, og, of)
+
+
+def test_config0_forces_and_groups(c0):
+    s, (total, groups, forces), (otot, og, of) = _compare(c0)
+    print("config 0 force rel RMS", rel_rms(forces, of), "groups", groups, og)
+    assert rel_rms(forces, of) < FORCE_RMS
+    for k in range(6):
+        scale = max(abs(og[k]), 1e-6 * abs(otot), 1.0)
+        assert abs(groups[k] - og[k]) / scale < ENERGY_REL, (k, groups[k], og[k])
+    s.close()
+
+
+def test_config1_triclinic_forces_and_groups(c1):
+    s, (total, groups, forces), (otot, og, of) = _compare(c1)
+    print("config 1 force rel RMS", rel_rms(forces, of), "groups", groups, og)
+    assert rel_rms(forces, of) < FORCE_RMS
+    for k in range(6):
+        scale = max(abs(og[k]), 1e-6 * abs(otot), 1.0)
+        assert abs(groups[k] - og[k]) / scale < ENERGY_REL, (k, groups[k], og[k])
+    s.close()
+
+
+def test_pair_census_matches_oracle(c0):
+    o = O.OracleSystem(c0.system)
+    d, _ = o.direct_space(c0.state.positions, c0.system.boxVectors, with_forces=False)
+    s = native.NativeSystem(c0.system)
+    s.set_positions(c0.state.positions)
+    pairs, tiles = s.pair_census()
+    assert pairs == d.pairs_geometric
+    assert tiles > 0
+    s.close()
+
+
+def test_repeated_evaluations_are_bit_identical(c1):
+    s = native.NativeSystem(c1.system)
+    s.set_positions(c1.state.positions)
+    _, g1, f1 = s.evaluate()
+    _, g2, f2 = s.evaluate()
+    assert np.array_equal(f1, f2)
+    assert np.array_equal(g1, g2)
+    s.close()
+
+
+def test_run_is_deterministic_and_advances_the_step(c0):
+    def trajectory():
+        s = native.NativeSystem(c0.system, seed=7)
+        s.set_positions(c0.state.positions)
+        s.set_velocities(np.zeros_like(c0.state.positions))
+        s.run(60)
+        out = s.positions(), s.velocities(), s.step
+        s.close()
+        return out
+
+    p1, v1, n1 = trajectory()
+    p2, v2, n2 = trajectory()
+    assert n1 == n2 == 60
+    assert np.array_equal(p1, p2)
+    assert np.array_equal(v1, v2)
+    assert np.all(np.isfinite(p1))
