@@ -156,6 +156,14 @@ md_err md_run(md_system *sys, int64_t steps);
 md_err md_get_step(md_system *sys, int64_t *step);
 md_err md_set_step(md_system *sys, int64_t step);
 
+/* constrainPositions(system, x, x, tol) and constrainVelocities(system, x, v,
+ * tol) (src/reference/constraints.ts:254-291) applied on the device to the
+ * current positions or velocities: SETTLE for rigid waters, Jacobi SHAKE or
+ * RATTLE for the other clusters.  The minimiser and the Maxwell-Boltzmann
+ * draw (src/reference/minimise.ts, velocities.ts:78-117) use them. */
+md_err md_constrain_positions(md_system *sys, double tolerance);
+md_err md_constrain_velocities(md_system *sys, double tolerance);
+
 /* StepLoop.verletFlags (src/gpu/loop.ts:941-960). */
 typedef struct {
   int64_t rebuilds;
@@ -165,6 +173,25 @@ typedef struct {
   int32_t overflowed;
 } md_list_flags;
 md_err md_list_flags_get(md_system *sys, md_list_flags *flags);
+
+/* StepLoop.profileStep (src/gpu/loop.ts:705-713, src/gpu/profile.ts:68-184):
+ * run `steps` steps launched phase by phase with HIP events recorded on the
+ * system's stream between phases; `ms[phase]` receives the mean milliseconds
+ * per step of each phase (MD_PHASE_COUNT entries). */
+enum {
+  MD_PHASE_PROLOGUE = 0, /* step counter, rebuild latch */
+  MD_PHASE_BUILD = 1,    /* cell sort + cluster-pair list (gated on the device) */
+  MD_PHASE_PACK = 2,     /* cluster-frame f32 coordinates */
+  MD_PHASE_PAIRS = 3,    /* LJ + erfc-Ewald pair kernel */
+  MD_PHASE_BONDED = 4,   /* bonds, angles, torsions, exceptions, restraints */
+  MD_PHASE_SPREAD = 5,   /* PME charge spreading */
+  MD_PHASE_FFT = 6,      /* PME forward FFT, convolution, inverse FFT */
+  MD_PHASE_GATHER = 7,   /* PME force interpolation */
+  MD_PHASE_INTEGRATE = 8,/* LFMiddle/Verlet + SETTLE/SHAKE/RATTLE + COM */
+  MD_PHASE_COUNT = 9
+};
+md_err md_profile_steps(md_system *sys, int32_t steps, double *ms);
+const char *md_phase_name(int32_t phase);
 
 /* Work counts the benchmark uses: in-cutoff pair count (exact, counted on the
  * device at the current positions) and cluster tiles computed. */

@@ -488,15 +488,33 @@ static void release(md_system *s) {
 
 // ---- steps and evaluations -------------------------------------------------
 
+void md_mark(md_system *s, int phase) {
+  if (!s->profiling) return;
+  hipEvent_t e;
+  MD_HIP(hipEventCreate(&e));
+  MD_HIP(hipEventRecord(e, s->stream));
+  s->prof_events.push_back(e);
+  s->prof_phase.push_back(phase);
+}
+
 static void launch_step(md_system *s, bool force_rebuild) {
+  md_mark(s, MD_PHASE_PROLOGUE);
   md_step_prologue(s, true, force_rebuild);
   if (s->has_nonbonded) {
+    md_mark(s, MD_PHASE_BUILD);
     md_nb_build(s, false);
+    md_mark(s, MD_PHASE_PACK);
     md_nb_pairs(s, false, false);
   }
+  md_mark(s, MD_PHASE_BONDED);
   md_bonded(s, false);
-  if (s->has_nonbonded) md_pme(s, false);
+  if (s->has_nonbonded) {
+    md_mark(s, MD_PHASE_SPREAD);
+    md_pme(s, false);
+  }
+  md_mark(s, MD_PHASE_INTEGRATE);
   md_integrate(s);
+  md_mark(s, -1);
 }
 
 static void check_control(md_system *s) {
@@ -726,6 +744,26 @@ md_err md_run(md_system *s, int64_t steps) {
   MD_API_END
 }
 
+md_err md_constrain_positions(md_system *s, double tolerance) {
+  MD_API_BEGIN
+  if (!s || !(tolerance > 0)) throw md_error(MD_ERR_INVALID_ARGUMENT, "bad arguments to md_constrain_positions");
+  MD_HIP(hipSetDevice(s->device));
+  md_constrain(s, 0, tolerance);
+  MD_HIP(hipStreamSynchronize(s->stream));
+  s->need_rebuild = true;
+  MD_API_END
+}
+
+md_err md_constrain_velocities(md_system *s, double tolerance) {
+  MD_API_BEGIN
+  if (!s || !(tolerance > 0)) throw md_error(MD_ERR_INVALID_ARGUMENT, "bad arguments to md_constrain_velocities");
+  MD_HIP(hipSetDevice(s->device));
+  md_constrain(s, 1, tolerance);
+  MD_HIP(hipStreamSynchronize(s->stream));
+  s->momentum_dirty = true;
+  MD_API_END
+}
+
 md_err md_get_step(md_system *s, int64_t *step) {
   MD_API_BEGIN
   if (!s || !step) throw md_error(MD_ERR_INVALID_ARGUMENT, "null argument");
@@ -755,6 +793,50 @@ md_err md_list_flags_get(md_system *s, md_list_flags *flags) {
   std::memcpy(&w, &c.max_disp2_bits[0], 4);
   flags->worst_displacement = std::sqrt((double)w);
   flags->overflowed = (int)c.overflow;
+  MD_API_END
+}
+
+const char *md_phase_name(int32_t phase) {
+  static const char *names[MD_PHASE_COUNT] = {"prologue", "list_build", "pack",   "pairs",    "bonded",
+                                              "pme_spread", "pme_fft_convolve", "pme_gather", "integrate"};
+  return phase >= 0 && phase < MD_PHASE_COUNT ? names[phase] : "";
+}
+
+md_err md_profile_steps(md_system *s, int32_t steps, double *ms) {
+  MD_API_BEGIN
+  if (!s || !ms || steps <= 0) throw md_error(MD_ERR_INVALID_ARGUMENT, "bad arguments to md_profile_steps");
+  MD_HIP(hipSetDevice(s->device));
+  for (int k = 0; k < MD_PHASE_COUNT; ++k) ms[k] = 0.0;
+  MD_HIP(hipMemcpyAsync(&s->ctl->step_next, &s->steps_done, sizeof(int64_t), hipMemcpyHostToDevice, s->stream));
+  if (s->momentum_dirty) {
+    md_momentum(s);
+    s->momentum_dirty = false;
+  }
+  md_zero_forces(s, false);
+  for (int r = 0; r < steps; ++r) {
+    s->profiling = true;
+    try {
+      launch_step(s, s->need_rebuild);
+    } catch (...) {
+      s->profiling = false;
+      throw;
+    }
+    s->profiling = false;
+    s->need_rebuild = false;
+    MD_HIP(hipStreamSynchronize(s->stream));
+    for (size_t k = 0; k + 1 < s->prof_events.size(); ++k) {
+      float t = 0.0f;
+      MD_HIP(hipEventElapsedTime(&t, s->prof_events[k], s->prof_events[k + 1]));
+      int ph = s->prof_phase[k];
+      if (ph >= 0 && ph < MD_PHASE_COUNT) ms[ph] += t;
+    }
+    for (hipEvent_t e : s->prof_events) (void)hipEventDestroy(e);
+    s->prof_events.clear();
+    s->prof_phase.clear();
+  }
+  for (int k = 0; k < MD_PHASE_COUNT; ++k) ms[k] /= steps;
+  s->steps_done += steps;
+  check_control(s);
   MD_API_END
 }
 

@@ -128,6 +128,10 @@ struct md_system {
   int64_t steps_done = 0;
   bool need_rebuild = true;
   bool momentum_dirty = true;
+  // profiling: events recorded by md_mark while `profiling` is set
+  bool profiling = false;
+  std::vector<hipEvent_t> prof_events;
+  std::vector<int> prof_phase;
 
   std::vector<void *> allocations;
 };
@@ -144,6 +148,10 @@ T *md_alloc(md_system *s, size_t count) {
   return static_cast<T *>(p);
 }
 
+// Profiling phases (mdcore.h MD_PHASE_*): a launcher calls md_mark when it
+// starts a phase; outside md_profile_steps it does nothing.
+void md_mark(md_system *s, int phase);
+
 // ---- launchers (each in its own .hip file) ----------------------------------
 void md_nb_setup(md_system *s);                 // nonbonded.hip
 void md_nb_free(md_system *s);
@@ -159,3 +167,4 @@ void md_pme(md_system *s, bool energy);
 void md_step_prologue(md_system *s, bool advance, bool force_rebuild);  // integrate.hip
 void md_integrate(md_system *s);
 void md_momentum(md_system *s);
+void md_constrain(md_system *s, int what, double tolerance);  // 0 positions, 1 velocities

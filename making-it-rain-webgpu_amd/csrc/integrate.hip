@@ -449,6 +449,65 @@ __global__ void __launch_bounds__(256) k_integrate(md_int_args A) {
   md_reduce_out(A, step, mom, worst);
 }
 
+// Constraint projection outside a step (src/reference/constraints.ts:254-291):
+// WHAT 0 projects the positions onto the constraints with themselves as the
+// reference (constrainPositions(x, x)); WHAT 1 removes every constrained
+// relative velocity (constrainVelocities).
+template <int KIND, int MAXA, int WHAT>
+__global__ void __launch_bounds__(256) k_constrain(md_int_args A) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int count = KIND == 1 ? A.n_waters : A.n_ccl;
+  if (t >= count) return;
+  md_item<MAXA> it;
+  int nc = 0;
+  int4 cons[KIND == 2 ? MD_MAX_CLUSTER_CONSTRAINTS : 1];
+  if (KIND == 1) {
+    int4 w = A.water_atoms[t];
+    it.n = 3;
+    it.atom[0] = w.x;
+    it.atom[1] = w.y;
+    it.atom[2] = w.z;
+  } else {
+    it.n = A.ccl_natoms[t];
+    for (int a = 0; a < MAXA; ++a) it.atom[a] = a < it.n ? A.ccl_atoms[t * MD_MAX_CLUSTER_ATOMS + a] : 0;
+    nc = A.ccl_ncons[t];
+    for (int c = 0; c < (KIND == 2 ? MD_MAX_CLUSTER_CONSTRAINTS : 1); ++c)
+      cons[c] = c < nc ? A.ccl_cons[t * MD_MAX_CLUSTER_CONSTRAINTS + c] : make_int4(0, 0, 0, 0);
+  }
+  for (int a = 0; a < it.n; ++a) {
+    int i = it.atom[a];
+    it.im[a] = A.inv_mass[i];
+    for (int d = 0; d < 3; ++d) {
+      it.x[a][d] = it.tr[a][d] = A.pos[d * A.np + i];
+      it.v[a][d] = A.vel[d * A.np + i];
+    }
+  }
+  if constexpr (KIND == 1) {
+    float m3[3];
+    for (int a = 0; a < 3; ++a) m3[a] = A.mass[it.atom[a]];
+    if (WHAT == 0) {
+      float2 dd = A.water_dist[t];
+      md_settle_positions(it, dd.x, dd.y, m3);
+    } else {
+      md_settle_velocities(it, m3);
+    }
+  } else {
+    if (WHAT == 0)
+      md_shake(A, it, nc, cons);
+    else
+      md_rattle(A, it, nc, cons);
+  }
+  for (int a = 0; a < it.n; ++a) {
+    int i = it.atom[a];
+    for (int d = 0; d < 3; ++d) {
+      if (WHAT == 0)
+        A.pos[d * A.np + i] = it.tr[a][d];
+      else
+        A.vel[d * A.np + i] = it.v[a][d];
+    }
+  }
+}
+
 // start of a step: advance the device step counter, latch the rebuild
 // request, clear the momentum slot this step's integrator fills
 __global__ void k_step_prologue(md_control *ctl, int advance, int force_rebuild) {
@@ -533,6 +592,33 @@ void md_integrate(md_system *s) {
       k_integrate<2, 4><<<(A.n_ccl + 255) / 256, 256, 0, st>>>(A);
     else
       k_integrate<2, 8><<<(A.n_ccl + 255) / 256, 256, 0, st>>>(A);
+  }
+  MD_HIP(hipGetLastError());
+}
+
+void md_constrain(md_system *s, int what, double tolerance) {
+  md_int_args A = int_args(s);
+  A.tol = tolerance;
+  hipStream_t st = s->stream;
+  if (A.n_waters) {
+    if (what == 0)
+      k_constrain<1, 3, 0><<<(A.n_waters + 255) / 256, 256, 0, st>>>(A);
+    else
+      k_constrain<1, 3, 1><<<(A.n_waters + 255) / 256, 256, 0, st>>>(A);
+  }
+  if (A.n_ccl) {
+    int g = (A.n_ccl + 255) / 256;
+    if (s->max_cluster_atoms <= 4) {
+      if (what == 0)
+        k_constrain<2, 4, 0><<<g, 256, 0, st>>>(A);
+      else
+        k_constrain<2, 4, 1><<<g, 256, 0, st>>>(A);
+    } else {
+      if (what == 0)
+        k_constrain<2, 8, 0><<<g, 256, 0, st>>>(A);
+      else
+        k_constrain<2, 8, 1><<<g, 256, 0, st>>>(A);
+    }
   }
   MD_HIP(hipGetLastError());
 }

@@ -859,6 +859,7 @@ void md_nb_pairs(md_system *s, bool energy, bool census) {
   int gs = (s->nb.nslots + 255) / 256;
   k_nb_pack<<<gs, 256, 0, st>>>(s->nb.nslots, s->np, s->slot_atom, s->pos, s->slot_offset, s->atom_param,
                                 s->slot_xyzq);
+  md_mark(s, MD_PHASE_PAIRS);
   md_pair_args A;
   A.segments = B.segments;
   A.entry_j = B.entry_j;

@@ -495,6 +495,7 @@ void md_pme(md_system *s, bool energy) {
   size_t ly = sizeof(float2) * 2 * PME_BATCH * P.ny;
   size_t lx = sizeof(float2) * 2 * PME_BATCH * P.nx;
   int kzb = (P.nzc + PME_BATCH - 1) / PME_BATCH;
+  md_mark(s, MD_PHASE_FFT);
   k_pme_z_forward<<<(rows + PME_BATCH - 1) / PME_BATCH, 256, lz, st>>>(P, s->pme_grid, s->pme_cgrid);
   k_pme_y<<<P.nx * kzb, 256, ly, st>>>(P, s->pme_cgrid, -1.0f);
   if (energy)
@@ -503,6 +504,7 @@ void md_pme(md_system *s, bool energy) {
     k_pme_x_convolve<false><<<P.ny * kzb, 256, lx, st>>>(P, s->pme_cgrid, s->energy);
   k_pme_y<<<P.nx * kzb, 256, ly, st>>>(P, s->pme_cgrid, 1.0f);
   k_pme_z_inverse<<<(rows + PME_BATCH - 1) / PME_BATCH, 256, lz, st>>>(P, s->pme_cgrid, s->pme_phi);
+  md_mark(s, MD_PHASE_GATHER);
   k_pme_gather<<<(n + 255) / 256, 256, 0, st>>>(n, s->np, s->pos, s->atom_param, P, s->pme_phi, s->force);
   MD_HIP(hipGetLastError());
 }

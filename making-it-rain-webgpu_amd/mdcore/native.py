@@ -131,6 +131,11 @@ def library():
         L.md_probe_fixed_point.argtypes = [P(C.c_float), C.c_int32, P(C.c_int64), P(C.c_float)]
         L.md_probe_thermostat.argtypes = [C.c_uint32, C.c_uint32, C.c_int32, P(C.c_uint32), P(C.c_float)]
         L.md_inject_fault.argtypes = [C.c_void_p, C.c_int32]
+        L.md_profile_steps.argtypes = [C.c_void_p, C.c_int32, _dp]
+        L.md_phase_name.argtypes = [C.c_int32]
+        L.md_phase_name.restype = C.c_char_p
+        L.md_constrain_positions.argtypes = [C.c_void_p, C.c_double]
+        L.md_constrain_velocities.argtypes = [C.c_void_p, C.c_double]
         _lib = L
     return _lib
 
@@ -313,6 +318,21 @@ class NativeSystem:
 
     def inject_fault(self, after: int):
         check(library().md_inject_fault(self.handle, int(after)))
+
+    def profile_steps(self, steps: int) -> dict:
+        """Mean milliseconds per step of each phase, timed with HIP events."""
+        ms = np.zeros(PHASE_COUNT)
+        check(library().md_profile_steps(self.handle, int(steps), _d(ms)))
+        return {library().md_phase_name(k).decode(): float(ms[k]) for k in range(PHASE_COUNT)}
+
+    def constrain_positions(self, tolerance: float):
+        check(library().md_constrain_positions(self.handle, float(tolerance)))
+
+    def constrain_velocities(self, tolerance: float):
+        check(library().md_constrain_velocities(self.handle, float(tolerance)))
+
+
+PHASE_COUNT = 9
 
 
 def probe_fixed_point(values):
