@@ -75,6 +75,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--minimise-iterations", type=int, default=300)
+    ap.add_argument("--max-batch", type=int, default=0,
+                    help="steps per captured HIP graph (0: 50; 1: launch every step directly)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build is single-GPU (asked for --gpus {args.gpus}). "
@@ -91,7 +93,8 @@ def main():
     log(f"built {n} atoms ({built.description['shape']}, {built.description['width_nm']:.3f} nm) "
         f"in {time.perf_counter() - t:.1f} s")
     engine = HipEngine(built.system, built.state, RunOptions(timeStep=0.002, temperature=300.0, friction=1.0,
-                                                            constraintTolerance=1e-6, seed=2026))
+                                                            constraintTolerance=1e-6, seed=2026,
+                                                            maxBatch=args.max_batch))
     t = time.perf_counter()
     from mdcore.engine import minimise
     res = minimise(engine, args.minimise_iterations, 10.0)

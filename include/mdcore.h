@@ -114,6 +114,9 @@ typedef struct {
   double verlet_skin;          /* nm; < 0 picks the default (0.15, nonbonded.ts:227) */
   int32_t mesh[3];             /* PME mesh override; 0 = from the system or the box */
   int32_t device;              /* HIP device ordinal */
+  int32_t max_batch;           /* steps per captured HIP graph (maxBatch,
+                                  src/gpu/loop.ts:151-158); 0 = 50, 1 = every
+                                  step launched directly, no graph */
 } md_run_options;
 
 typedef struct md_system md_system;
@@ -141,6 +144,10 @@ md_err md_get_velocities(md_system *sys, double *velocities);
  * is recomputed; the mesh size stays fixed, as upstream. */
 md_err md_set_box(md_system *sys, const double box[9]);
 md_err md_get_box(md_system *sys, double box[9]);
+
+/* StepLoop.mesh (src/gpu/loop.ts:973-981): the PME mesh chosen at creation, all
+ * zero without a NonbondedForce. */
+md_err md_get_mesh(md_system *sys, int32_t mesh[3]);
 
 /* StepLoop.potential / evaluateForces / report (src/gpu/loop.ts:761-930):
  * forces (kJ/mol/nm, may be NULL) and per-group energies (MD_GROUP_COUNT,
@@ -201,6 +208,11 @@ md_err md_pair_census(md_system *sys, int64_t *pairs_within_cutoff, int64_t *til
  * fixed-point twin): run the device primitives on host-given inputs. */
 md_err md_probe_fixed_point(const float *values, int32_t n, int64_t *fixed, float *back);
 md_err md_probe_thermostat(uint32_t seed, uint32_t step, int32_t n, uint32_t *bits, float *normals);
+/* The order in which the neighbour search sorts cells (host only, no
+ * device): cells[k] = (x * dims[1] + y) * dims[2] + z of the k-th cell of a
+ * dims[0] x dims[1] x dims[2] grid.  Stands in for the counting sort's cell
+ * numbering (src/gpu/cell-list.ts:136-155); `cells` holds the grid's size. */
+md_err md_probe_cell_order(const int32_t dims[3], int32_t *cells);
 
 /* Fault injection for host-side failure tests: the next `after` kernel
  * launches succeed, then every call returns MD_ERR_FAULT_INJECTED until

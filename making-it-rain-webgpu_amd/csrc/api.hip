@@ -268,6 +268,8 @@ static void create(md_system *s, const md_system_desc *d, const md_run_options *
   if (n <= 0) throw md_error(MD_ERR_INVALID_ARGUMENT, "a system needs at least one particle");
   s->opt = *o;
   if (s->opt.time_step <= 0) throw md_error(MD_ERR_INVALID_ARGUMENT, "time_step must be positive");
+  if (s->opt.max_batch < 0 || s->opt.max_batch > 100000)
+    throw md_error(MD_ERR_INVALID_ARGUMENT, "max_batch must be between 0 and 100000");
   if (s->opt.constraint_tolerance <= 0) s->opt.constraint_tolerance = 1e-6;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
@@ -520,6 +522,10 @@ static void launch_step(md_system *s, bool force_rebuild) {
 static void check_control(md_system *s) {
   md_control c;
   MD_HIP(hipMemcpy(&c, s->ctl, sizeof c, hipMemcpyDeviceToHost));
+  if (c.overflow == 2)
+    throw md_error(MD_ERR_CUTOFF_TOO_LARGE,
+                   "the list radius plus the cluster extents exceeds the narrowest box width: the box is too "
+                   "small for this cutoff");
   if (c.overflow)
     throw md_error(MD_ERR_LIST_OVERFLOW,
                    "the cluster-pair list outgrew its capacity (the system compressed far beyond its starting "
@@ -551,12 +557,13 @@ static void run_steps(md_system *s, int64_t steps) {
     s->need_rebuild = false;
     --left;
   }
-  if (left >= STRIDE_STEPS && !s->stride_exec) {
+  const int stride = s->opt.max_batch > 0 ? s->opt.max_batch : STRIDE_STEPS;
+  if (stride > 1 && left >= stride && !s->stride_exec) {
     MD_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < STRIDE_STEPS; ++k) launch_step(s, false);
+    for (int k = 0; k < stride; ++k) launch_step(s, false);
     MD_HIP(hipStreamEndCapture(s->stream, &s->stride_graph));
     MD_HIP(hipGraphInstantiate(&s->stride_exec, s->stride_graph, nullptr, nullptr, 0));
-    s->stride_steps = STRIDE_STEPS;
+    s->stride_steps = stride;
   }
   while (s->stride_exec && left >= s->stride_steps) {
     MD_HIP(hipGraphLaunch(s->stride_exec, s->stream));
@@ -730,6 +737,13 @@ md_err md_get_box(md_system *s, double box[9]) {
   MD_API_END
 }
 
+md_err md_get_mesh(md_system *s, int32_t mesh[3]) {
+  MD_API_BEGIN
+  if (!s || !mesh) throw md_error(MD_ERR_INVALID_ARGUMENT, "null argument");
+  for (int a = 0; a < 3; ++a) mesh[a] = s->has_nonbonded ? s->grid[a] : 0;
+  MD_API_END
+}
+
 md_err md_evaluate(md_system *s, double *forces, double *groups, double *total) {
   MD_API_BEGIN
   if (!s) throw md_error(MD_ERR_INVALID_ARGUMENT, "null handle");
@@ -890,6 +904,16 @@ md_err md_probe_thermostat(uint32_t seed, uint32_t step, int32_t n, uint32_t *bi
   MD_HIP(hipMemcpy(normals, dn, sizeof(float) * 3 * n, hipMemcpyDeviceToHost));
   hipFree(db);
   hipFree(dn);
+  MD_API_END
+}
+
+md_err md_probe_cell_order(const int32_t dims[3], int32_t *cells) {
+  MD_API_BEGIN
+  if (!dims || !cells || dims[0] < 1 || dims[1] < 1 || dims[2] < 1 ||
+      (int64_t)dims[0] * dims[1] * dims[2] > (1 << 26))
+    throw md_error(MD_ERR_INVALID_ARGUMENT, "bad cell grid");
+  std::vector<int> walk = md_cell_walk(dims[0], dims[1], dims[2]);
+  std::copy(walk.begin(), walk.end(), cells);
   MD_API_END
 }
 

@@ -43,6 +43,8 @@ struct md_control {
   uint32_t cluster_reach_bits[2];  // largest cluster radius, super-cluster reach
   int64_t momentum[2][3];          // sum m v in Q32.32, by step parity
   int64_t pair_census;             // in-cutoff pairs, when counted
+  int32_t segment_cursor;          // next list segment for the pair kernel's waves
+  int32_t pad_;
 };
 
 struct md_nb_params {
@@ -154,6 +156,7 @@ void md_mark(md_system *s, int phase);
 
 // ---- launchers (each in its own .hip file) ----------------------------------
 void md_nb_setup(md_system *s);                 // nonbonded.hip
+std::vector<int> md_cell_walk(int W, int H, int D);  // nonbonded.hip: cell sort order
 void md_nb_free(md_system *s);
 void md_nb_build(md_system *s, bool forced);    // rebuild kernels, device-gated
 void md_nb_pairs(md_system *s, bool energy, bool census);

@@ -278,11 +278,16 @@ __device__ __forceinline__ bool md_boxes_within(float dx, float dy, float dz, fl
 // minimum image of the centre difference, and -- when the box is narrow
 // compared with the list radius plus the cluster sizes -- its 26 neighbours.
 // Emits, in a fixed order, the image offsets (as lattice multiples added to
-// the reduced difference) that pass the box test.
+// the reduced difference) that pass the box test.  For a cluster of the
+// super-cluster itself (`own`) the images are centred on the unshifted
+// cluster instead, so that code 0 is the zero shift and the codes of a
+// shift and of its negative have opposite signs: a super-cluster that spans
+// a periodic boundary would otherwise centre them elsewhere, and keeping
+// the positive codes would then keep both of some +/- pairs.
 template <typename F>
 __device__ void md_for_each_image(const md_build_ctx &c, int64_t dx, int64_t dy, int64_t dz, bool enumerate,
-                                  F &&visit) {
-  md_min_image_fixed(dx, dy, dz, c.box);
+                                  bool own, F &&visit) {
+  if (!own) md_min_image_fixed(dx, dy, dz, c.box);
   if (!enumerate) {
     visit(dx, dy, dz, 0);
     return;
@@ -322,9 +327,11 @@ __device__ void md_candidate(const md_build_ctx &c, const md_list_ptrs &L, int f
                              const float4 *ch, F &&emit) {
   int64_t jx = L.cluster_center[J], jy = L.cluster_center[c.nclusters + J], jz = L.cluster_center[2 * c.nclusters + J];
   float4 jh = L.cluster_half[J];
-  // the image is chosen against S's first cluster and shared by the entry
+  // the image is chosen against S's first cluster and shared by the entry;
+  // S's own clusters keep their unshifted frame (see md_for_each_image)
   int64_t dx = jx - cx[0], dy = jy - cy[0], dz = jz - cz[0];
-  md_for_each_image(c, dx, dy, dz, enumerate, [&](int64_t ex, int64_t ey, int64_t ez, int code) {
+  bool own = J < first + 8;
+  md_for_each_image(c, dx, dy, dz, enumerate, own, [&](int64_t ex, int64_t ey, int64_t ez, int code) {
     // the lattice vector that moves J into this image
     int64_t shx = ex - dx, shy = ey - dy, shz = ez - dz;
     bool zero = shx == 0 && shy == 0 && shz == 0;
@@ -357,6 +364,9 @@ __global__ void __launch_bounds__(64) k_nb_list(md_build_ctx c, md_list_ptrs L, 
   if (S >= c.nsuper) return;
   float reach = __uint_as_float(ctl->cluster_reach_bits[0]) + __uint_as_float(ctl->cluster_reach_bits[1]);
   bool enumerate = c.list_cutoff + reach > c.half_min_width;
+  // the 27 images around the minimum image hold every image within the
+  // narrowest width of the first cluster; beyond that, refuse
+  if (PASS == 0 && S == 0 && lane == 0 && c.list_cutoff + reach > 2.0f * c.half_min_width) ctl->overflow = 2;
   int first = S * 8;
   int nlocal = min(8, c.nclusters - first);
   int64_t cx[8], cy[8], cz[8];
@@ -445,43 +455,46 @@ __global__ void __launch_bounds__(64) k_nb_masks(md_build_ctx c, md_list_ptrs L,
                                                  const int *__restrict__ excl_list, const int *totals,
                                                  const md_control *ctl, int forced) {
   if (!md_build_gate(ctl, forced)) return;
-  int e = blockIdx.x;
-  if (e >= totals[0]) return;
+  int total = totals[0];
   int lane = threadIdx.x, il = lane >> 3, jl = lane & 7;
-  // which super-cluster owns entry e: binary search the starts
-  int lo = 0, hi = c.nsuper;
-  while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
-    if (L.super_count[mid] <= e) lo = mid; else hi = mid;
-  }
-  int S = lo;
-  int J = L.entry_j[e];
-  int meta = L.entry_imask[e];
-  int jslot = J * MD_CLUSTER + jl;
-  int ja = slot_atom[jslot];
-  for (int q = 0; q < 8; ++q) {
-    int I = S * 8 + q;
-    int islot = I * MD_CLUSTER + il;
-    int ia = I < c.nclusters ? slot_atom[islot] : -1;
-    bool ok = ((meta >> q) & 1) && ia >= 0 && ja >= 0;
-    if (ok && ((meta >> (8 + q)) & 1)) ok = jl > il;  // same cluster, zero image
-    if (ok) {
-      for (int k = excl_start[ia]; k < excl_start[ia + 1]; ++k)
-        if (excl_list[k] == ja) {
-          ok = false;
-          break;
-        }
+  // one wave per entry, striding over the entries the list holds
+  for (int e = blockIdx.x; e < total; e += gridDim.x) {
+    // which super-cluster owns entry e: binary search the starts
+    int lo = 0, hi = c.nsuper;
+    while (hi - lo > 1) {
+      int mid = (lo + hi) >> 1;
+      if (L.super_count[mid] <= e) lo = mid; else hi = mid;
     }
-    uint64_t m = __ballot(ok);
-    if (lane == 0) L.entry_excl[(size_t)e * 8 + q] = m;
+    int S = lo;
+    int J = L.entry_j[e];
+    int meta = L.entry_imask[e];
+    int jslot = J * MD_CLUSTER + jl;
+    int ja = slot_atom[jslot];
+    for (int q = 0; q < 8; ++q) {
+      int I = S * 8 + q;
+      int islot = I * MD_CLUSTER + il;
+      int ia = I < c.nclusters ? slot_atom[islot] : -1;
+      bool ok = ((meta >> q) & 1) && ia >= 0 && ja >= 0;
+      if (ok && ((meta >> (8 + q)) & 1)) ok = jl > il;  // same cluster, zero image
+      if (ok) {
+        for (int k = excl_start[ia]; k < excl_start[ia + 1]; ++k)
+          if (excl_list[k] == ja) {
+            ok = false;
+            break;
+          }
+      }
+      uint64_t m = __ballot(ok);
+      if (lane == 0) L.entry_excl[(size_t)e * 8 + q] = m;
+    }
   }
 }
 
 // Per step: each slot's position as f32 in its super-cluster's frame.
 __global__ void k_nb_pack(int nslots, int np, const int *__restrict__ slot_atom, const int64_t *__restrict__ pos,
                           const int64_t *__restrict__ slot_offset, const float4 *__restrict__ atom_param,
-                          float4 *slot_xyzq) {
+                          float4 *slot_xyzq, md_control *ctl) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s == 0) ctl->segment_cursor = 0;  // the pair kernel that follows hands out segments from 0
   if (s >= nslots) return;
   int a = slot_atom[s];
   if (a < 0) {
@@ -512,6 +525,7 @@ struct md_pair_args {
   int64_t *force;
   int64_t *energy;
   int64_t *census;
+  int32_t *segment_cursor;
   int np, nclusters;
   float cutoff2, alpha, beta, exp_scale;
 };
@@ -547,13 +561,13 @@ __device__ __forceinline__ float md_pair(float dx, float dy, float dz, float qi,
   return fs;
 }
 
+// One segment: up to 8 list entries of super-cluster S against their j
+// clusters.
 template <bool ENERGY>
-__global__ void __launch_bounds__(64) k_nb_pairs(md_pair_args A) {
-  int seg = blockIdx.x;
-  if (seg >= A.totals[1]) return;
+__device__ __forceinline__ void md_pair_segment(const md_pair_args &A, int seg, int lane, int il, int jl,
+                                                int64_t &efix, int &count) {
   int4 sg = A.segments[seg];
   int S = sg.x, eb = sg.y, ee = sg.z;
-  int lane = threadIdx.x, il = lane >> 3, jl = lane & 7;
   // i data: lane's atom in each of S's 8 clusters (8 clusters x 8 atoms = 64
   // slots; lane il*8+jl reads slot S*64 + q*8 + il)
   float xi[8], yi[8], zi[8], qi[8], si[8], ei[8];
@@ -572,7 +586,6 @@ __global__ void __launch_bounds__(64) k_nb_pairs(md_pair_args A) {
     fx[q] = fy[q] = fz[q] = 0.0f;
   }
   float en = 0.0f;
-  int count = 0;
   for (int e = eb; e < ee; ++e) {
     int J = A.entry_j[e];
     int meta = A.entry_imask[e];
@@ -633,8 +646,27 @@ __global__ void __launch_bounds__(64) k_nb_pairs(md_pair_args A) {
       }
     }
   }
+  if (ENERGY) efix += md_to_fixed(en);
+}
+
+template <bool ENERGY>
+__global__ void __launch_bounds__(64) k_nb_pairs(md_pair_args A) {
+  const int nseg = A.totals[1];
+  const int lane = threadIdx.x, il = lane >> 3, jl = lane & 7;
+  // each segment's energy is rounded to Q32.32 on its own, so the sum does
+  // not depend on which wave took which segment
+  int64_t efix = 0;
+  int count = 0;
+  // a resident grid of waves takes segments from a counter until none are left
+  for (;;) {
+    int seg = 0;
+    if (lane == 0) seg = atomicAdd(A.segment_cursor, 1);
+    seg = __shfl(seg, 0, 64);
+    if (seg >= nseg) break;
+    md_pair_segment<ENERGY>(A, seg, lane, il, jl, efix, count);
+  }
   if (ENERGY) {
-    int64_t ef = md_wave_sum_i64(md_to_fixed(en));
+    int64_t ef = md_wave_sum_i64(efix);
     int64_t cf = md_wave_sum_i64((int64_t)count);
     if (lane == 0) {
       md_atomic_add_i64(A.energy + MD_E_PAIR, ef);
@@ -647,32 +679,80 @@ __global__ void __launch_bounds__(64) k_nb_pairs(md_pair_args A) {
 // Host side
 // ---------------------------------------------------------------------------
 
-// Skilling's transpose form of the Hilbert index (J. Skilling, "Programming
-// the Hilbert curve", AIP Conf. Proc. 707, 381 (2004)).
-static uint64_t hilbert_index(uint32_t x, uint32_t y, uint32_t z, int bits) {
-  uint32_t X[3] = {x, y, z};
-  uint32_t M = 1u << (bits - 1);
-  for (uint32_t Q = M; Q > 1; Q >>= 1) {
-    uint32_t P = Q - 1;
-    for (int i = 0; i < 3; i++) {
-      if (X[i] & Q) {
-        X[0] ^= P;
-      } else {
-        uint32_t t = (X[0] ^ X[i]) & P;
-        X[0] ^= t;
-        X[i] ^= t;
-      }
-    }
+// Generalized Hilbert walk over a W x H x D block of cells (the "gilbert"
+// construction, J. Cerveny 2018): the block is split along its long axes
+// into sub-blocks walked with rotated axes, each starting where the last
+// ended.  Every cell is visited once; with even W, H and D every step goes
+// to a face neighbour, so runs of consecutive cells -- the clusters and
+// super-clusters of the list -- stay compact and never wrap the box.
+namespace {
+struct md_v3 {
+  int x, y, z;
+};
+inline md_v3 operator+(md_v3 a, md_v3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline md_v3 operator-(md_v3 a, md_v3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline md_v3 operator-(md_v3 a) { return {-a.x, -a.y, -a.z}; }
+inline int md_sgn(int v) { return (v > 0) - (v < 0); }
+inline int md_floor_half(int v) { return v >= 0 ? v / 2 : -((1 - v) / 2); }
+inline md_v3 md_half(md_v3 a) { return {md_floor_half(a.x), md_floor_half(a.y), md_floor_half(a.z)}; }
+inline md_v3 md_dir(md_v3 a) { return {md_sgn(a.x), md_sgn(a.y), md_sgn(a.z)}; }
+inline int md_len(md_v3 a) { return std::abs(a.x + a.y + a.z); }
+
+// walk the block at p spanned by the axis-aligned vectors a (major), b, c
+void md_walk(md_v3 p, md_v3 a, md_v3 b, md_v3 c, std::vector<md_v3> &out) {
+  const int w = md_len(a), h = md_len(b), d = md_len(c);
+  const md_v3 da = md_dir(a), db = md_dir(b), dc = md_dir(c);
+  if (h == 1 && d == 1) {
+    for (int i = 0; i < w; ++i, p = p + da) out.push_back(p);
+    return;
   }
-  for (int i = 1; i < 3; i++) X[i] ^= X[i - 1];
-  uint32_t t = 0;
-  for (uint32_t Q = M; Q > 1; Q >>= 1)
-    if (X[2] & Q) t ^= Q - 1;
-  for (int i = 0; i < 3; i++) X[i] ^= t;
-  uint64_t h = 0;
-  for (int b = bits - 1; b >= 0; --b)
-    for (int i = 0; i < 3; i++) h = (h << 1) | ((X[i] >> b) & 1u);
-  return h;
+  if (w == 1 && d == 1) {
+    for (int i = 0; i < h; ++i, p = p + db) out.push_back(p);
+    return;
+  }
+  if (w == 1 && h == 1) {
+    for (int i = 0; i < d; ++i, p = p + dc) out.push_back(p);
+    return;
+  }
+  md_v3 a2 = md_half(a), b2 = md_half(b), c2 = md_half(c);
+  if ((md_len(a2) & 1) && w > 2) a2 = a2 + da;  // even halves where possible
+  if ((md_len(b2) & 1) && h > 2) b2 = b2 + db;
+  if ((md_len(c2) & 1) && d > 2) c2 = c2 + dc;
+  if (2 * w > 3 * h && 2 * w > 3 * d) {  // long in a: two halves along a
+    md_walk(p, a2, b, c, out);
+    md_walk(p + a2, a - a2, b, c, out);
+  } else if (3 * h > 4 * d) {  // flat in c: three blocks, c kept whole
+    md_walk(p, b2, c, a2, out);
+    md_walk(p + b2, a, b - b2, c, out);
+    md_walk(p + (a - da) + (b2 - db), -b2, c, -(a - a2), out);
+  } else if (3 * d > 4 * h) {  // flat in b: three blocks, b kept whole
+    md_walk(p, c2, a2, b, out);
+    md_walk(p + c2, a, b, c - c2, out);
+    md_walk(p + (a - da) + (c2 - dc), -c2, -(a - a2), b, out);
+  } else {  // the regular case: five blocks
+    md_walk(p, b2, c2, a2, out);
+    md_walk(p + b2, c, a2, b - b2, out);
+    md_walk(p + (b2 - db) + (c - dc), a, -b2, -(c - c2), out);
+    md_walk(p + (a - da) + b2 + (c - dc), -c, -(a - a2), b - b2, out);
+    md_walk(p + (a - da) + (b2 - db), -b2, c2, -(a - a2), out);
+  }
+}
+}  // namespace
+
+std::vector<int> md_cell_walk(int W, int H, int D) {
+  std::vector<md_v3> out;
+  out.reserve((size_t)W * H * D);
+  const md_v3 o{0, 0, 0}, X{W, 0, 0}, Y{0, H, 0}, Z{0, 0, D};
+  if (W >= H && W >= D)
+    md_walk(o, X, Y, Z, out);
+  else if (H >= W && H >= D)
+    md_walk(o, Y, X, Z, out);
+  else
+    md_walk(o, Z, X, Y, out);
+  std::vector<int> cells;
+  cells.reserve(out.size());
+  for (const md_v3 &v : out) cells.push_back((v.x * H + v.y) * D + v.z);
+  return cells;
 }
 
 struct md_nb_buffers {
@@ -685,6 +765,8 @@ struct md_nb_buffers {
   int *totals = nullptr;
   int nsuper = 0;
   int entry_capacity = 0, segment_capacity = 0;
+  int pair_grid[2] = {1, 1};  // resident waves of k_nb_pairs<false>, <true>
+  int mask_grid = 1;
   float half_min_width = 0, list_cutoff = 0;
 };
 
@@ -721,20 +803,16 @@ void md_nb_setup(md_system *s) {
     widths[2] = v / cross_norm(b, b + 3);
   }
   B.half_min_width = (float)(0.5 * std::min(widths[0], std::min(widths[1], widths[2])));
-  for (int d = 0; d < 3; ++d) nb.gdim[d] = std::max(1, (int)std::floor(widths[d] / 0.3));
+  // cells of at least 0.3 nm, an even number along each axis so that the
+  // sorting walk is continuous
+  for (int d = 0; d < 3; ++d) nb.gdim[d] = std::max(2, 2 * (int)std::floor(widths[d] / 0.6));
   nb.ncells = nb.gdim[0] * nb.gdim[1] * nb.gdim[2];
-  int bits = 1;
-  while ((1 << bits) < std::max(nb.gdim[0], std::max(nb.gdim[1], nb.gdim[2]))) ++bits;
-  std::vector<std::pair<uint64_t, int>> order(nb.ncells);
-  for (int a = 0; a < nb.gdim[0]; ++a)
-    for (int b = 0; b < nb.gdim[1]; ++b)
-      for (int c = 0; c < nb.gdim[2]; ++c) {
-        int cell = (a * nb.gdim[1] + b) * nb.gdim[2] + c;
-        order[cell] = {hilbert_index(a, b, c, bits), cell};
-      }
-  std::sort(order.begin(), order.end());
-  std::vector<int> rank(nb.ncells);
-  for (int k = 0; k < nb.ncells; ++k) rank[order[k].second] = k;
+  std::vector<int> walk = md_cell_walk(nb.gdim[0], nb.gdim[1], nb.gdim[2]);
+  if ((int)walk.size() != nb.ncells) throw md_error(MD_ERR_INVALID_ARGUMENT, "cell walk does not cover the grid");
+  std::vector<int> rank(nb.ncells, -1);
+  for (int k = 0; k < nb.ncells; ++k) rank[walk[k]] = k;
+  for (int k = 0; k < nb.ncells; ++k)
+    if (rank[k] < 0) throw md_error(MD_ERR_INVALID_ARGUMENT, "cell walk misses a cell");
 
   int np = s->np;
   s->cell_rank = md_alloc<int>(s, nb.ncells);
@@ -771,6 +849,15 @@ void md_nb_setup(md_system *s) {
   cap = std::min(cap, (long)B.nsuper * (long)nb.nclusters * 27 + 64);
   B.entry_capacity = (int)cap;
   B.segment_capacity = B.entry_capacity / MD_SEGMENT + B.nsuper + 64;
+  // the pair kernel keeps one resident grid busy from a segment counter; the
+  // exclusion masks stride a fixed grid over the entries
+  int cus = 1, per_cu[2] = {1, 1};
+  MD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));
+  MD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[0], k_nb_pairs<false>, 64, 0));
+  MD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[1], k_nb_pairs<true>, 64, 0));
+  for (int k = 0; k < 2; ++k)
+    B.pair_grid[k] = std::max(1, std::min(B.segment_capacity, std::max(1, cus) * std::max(1, per_cu[k])));
+  B.mask_grid = std::max(1, std::min(B.entry_capacity, std::max(1, cus) * 32));
   B.entry_j = md_alloc<int>(s, B.entry_capacity);
   B.entry_imask = md_alloc<int>(s, B.entry_capacity);
   B.entry_shift = md_alloc<float4>(s, B.entry_capacity);
@@ -848,7 +935,7 @@ void md_nb_build(md_system *s, bool forced) {
   k_nb_list_scan<<<1, 1024, 0, st>>>(B.nsuper, L, B.entry_capacity, B.segment_capacity, s->ctl, B.totals, f);
   k_nb_list<1><<<B.nsuper, 64, 0, st>>>(c, L, s->ctl, f);
   k_nb_segments<<<(B.nsuper + 63) / 64, 64, 0, st>>>(B.nsuper, L, B.entry_capacity, B.segment_capacity, s->ctl, f);
-  k_nb_masks<<<B.entry_capacity, 64, 0, st>>>(c, L, s->slot_atom, s->atom_slot, s->excl_start_d, s->excl_list_d,
+  k_nb_masks<<<B.mask_grid, 64, 0, st>>>(c, L, s->slot_atom, s->atom_slot, s->excl_start_d, s->excl_list_d,
                                               B.totals, s->ctl, f);
   MD_HIP(hipGetLastError());
 }
@@ -858,7 +945,7 @@ void md_nb_pairs(md_system *s, bool energy, bool census) {
   hipStream_t st = s->stream;
   int gs = (s->nb.nslots + 255) / 256;
   k_nb_pack<<<gs, 256, 0, st>>>(s->nb.nslots, s->np, s->slot_atom, s->pos, s->slot_offset, s->atom_param,
-                                s->slot_xyzq);
+                                s->slot_xyzq, s->ctl);
   md_mark(s, MD_PHASE_PAIRS);
   md_pair_args A;
   A.segments = B.segments;
@@ -873,6 +960,7 @@ void md_nb_pairs(md_system *s, bool energy, bool census) {
   A.force = s->force;
   A.energy = s->energy;
   A.census = &s->ctl->pair_census;
+  A.segment_cursor = &s->ctl->segment_cursor;
   A.np = s->np;
   A.nclusters = s->nb.nclusters;
   A.cutoff2 = s->nb.cutoff2;
@@ -880,9 +968,9 @@ void md_nb_pairs(md_system *s, bool energy, bool census) {
   A.beta = (float)(2.0 * s->alpha / std::sqrt(M_PI));
   A.exp_scale = (float)(-s->alpha * s->alpha / std::log(2.0));
   if (energy || census)
-    k_nb_pairs<true><<<B.segment_capacity, 64, 0, st>>>(A);
+    k_nb_pairs<true><<<B.pair_grid[1], 64, 0, st>>>(A);
   else
-    k_nb_pairs<false><<<B.segment_capacity, 64, 0, st>>>(A);
+    k_nb_pairs<false><<<B.pair_grid[0], 64, 0, st>>>(A);
   MD_HIP(hipGetLastError());
 }
 

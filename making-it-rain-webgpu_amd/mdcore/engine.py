@@ -16,6 +16,7 @@ from typing import Optional, Sequence
 
 import numpy as np
 
+from . import barostat as baro
 from . import native
 from .state import OpenMMState
 from .system import OpenMMSystem, reduceBox
@@ -42,6 +43,7 @@ class RunOptions:
     verletSkin: float = DEFAULT_VERLET_SKIN
     mesh: Optional[Sequence[int]] = None
     device: int = 0
+    maxBatch: int = 0  # steps per captured HIP graph; 0 = 50, 1 = no graphs
 
 
 @dataclass
@@ -114,7 +116,8 @@ class HipEngine:
         self._native = native.NativeSystem(
             system, time_step=o.timeStep, temperature=o.temperature, friction=o.friction,
             constraint_tolerance=o.constraintTolerance, seed=o.seed, integrator=o.integrator,
-            verlet_skin=o.verletSkin, mesh=o.mesh or (0, 0, 0), device=o.device, box=self._box)
+            verlet_skin=o.verletSkin, mesh=o.mesh or (0, 0, 0), device=o.device, box=self._box,
+            max_batch=o.maxBatch)
         self._native.set_positions(state.positions)
         v = state.velocities if state.velocities is not None else np.zeros_like(state.positions)
         self._native.set_velocities(v)
@@ -125,6 +128,14 @@ class HipEngine:
         cmm = system.force("CMMotionRemover")
         self.degreesOfFreedom = int(3 * np.count_nonzero(self._moving) - system.constraintCount
                                     - (3 if cmm is not None else 0))
+        # Monte Carlo barostat (src/gpu/loop.ts:602-685): host-driven moves
+        # between strides, deviates from their own Threefry domain
+        self.barostat = system.force("MonteCarloBarostat")
+        self.barostatState = baro.BarostatState.initial(self._box) if self.barostat is not None else None
+        self.barostatDraw = 0
+        self.barostatAttempts = 0
+        self.barostatAccepted = 0
+        self._molecules = None
 
     # -- ProtocolEngine ----------------------------------------------------
     @property
@@ -144,6 +155,11 @@ class HipEngine:
     def boxVectors(self):
         return tuple(tuple(float(x) for x in r) for r in self._native.box())
 
+    @property
+    def mesh(self):
+        """The PME mesh, fixed for the engine's life (src/gpu/loop.ts:973-981)."""
+        return self._native.mesh()
+
     def boxLengths(self):
         b = self._native.box()
         return (float(b[0, 0]), float(b[1, 1]), float(b[2, 2]))
@@ -162,7 +178,47 @@ class HipEngine:
         self._native.set_box(self._box)
 
     def run(self, steps: int) -> None:
-        self._native.run(int(steps))
+        """``steps`` steps; with a barostat, cut at each attempt so the move
+        happens on the step OpenMM puts it on (src/gpu/loop.ts:662-685)."""
+        steps = int(steps)
+        b = self.barostat
+        if b is None or b.frequency <= 0:
+            self._native.run(steps)
+            return
+        done = 0
+        while done < steps:
+            until = b.frequency - self.barostatState.step
+            chunk = min(max(1, until), steps - done)
+            self._native.run(chunk)
+            done += chunk
+            self._attempt_barostat(chunk)
+
+    def _attempt_barostat(self, steps_just_run: int) -> None:
+        if not baro.acts_after_steps(self.barostatState, self.barostat.frequency, steps_just_run):
+            return
+        if self._molecules is None:
+            self._molecules = baro.find_molecules(self.system)
+        seed = int(self.options.seed) & 0xFFFFFFFF
+
+        def uniform():
+            b0, _ = threefry2x32([self.barostatDraw & 0xFFFFFFFF], [seed], BAROSTAT_DOMAIN, 0)
+            self.barostatDraw += 1
+            return float(_unit(b0)[0])
+
+        def potential(trial, trial_box):
+            self.setBox(trial_box)
+            self._native.set_positions(trial)
+            total, _, _ = self._native.evaluate(False)
+            return total
+
+        res = baro.attempt(self.barostat, self.barostatState, self._box, self.positions(), potential, uniform,
+                           self._molecules)
+        # the trial is on the device either way: commit the winner
+        self.setBox(res.box)
+        self.setPositions(res.positions)
+        self.barostatAttempts += 1
+        if res.accepted:
+            self.barostatAccepted += 1
 
     def sample(self) -> dict:
         r = self.report()

@@ -94,7 +94,8 @@ class SystemDesc(C.Structure):
 class RunOptionsC(C.Structure):
     _fields_ = [("time_step", C.c_double), ("temperature", C.c_double), ("friction", C.c_double),
                 ("constraint_tolerance", C.c_double), ("seed", C.c_uint32), ("integrator", C.c_int32),
-                ("verlet_skin", C.c_double), ("mesh", C.c_int32 * 3), ("device", C.c_int32)]
+                ("verlet_skin", C.c_double), ("mesh", C.c_int32 * 3), ("device", C.c_int32),
+                ("max_batch", C.c_int32)]
 
 
 class ListFlags(C.Structure):
@@ -136,6 +137,8 @@ def library():
         L.md_phase_name.restype = C.c_char_p
         L.md_constrain_positions.argtypes = [C.c_void_p, C.c_double]
         L.md_constrain_velocities.argtypes = [C.c_void_p, C.c_double]
+        L.md_get_mesh.argtypes = [C.c_void_p, _ip]
+        L.md_probe_cell_order.argtypes = [_ip, _ip]
         _lib = L
     return _lib
 
@@ -223,7 +226,7 @@ class NativeSystem:
 
     def __init__(self, system, time_step=0.002, temperature=300.0, friction=1.0,
                  constraint_tolerance=1e-6, seed=0, integrator="LangevinMiddle", verlet_skin=-1.0,
-                 mesh=(0, 0, 0), device=0, box=None):
+                 mesh=(0, 0, 0), device=0, box=None, max_batch=0):
         L = library()
         desc, keep = describe(system, box)
         o = RunOptionsC()
@@ -235,6 +238,7 @@ class NativeSystem:
         for k in range(3):
             o.mesh[k] = int(mesh[k])
         o.device = device
+        o.max_batch = int(max_batch)
         h = C.c_void_p()
         check(L.md_system_create(C.byref(desc), C.byref(o), C.byref(h)))
         self._h = h
@@ -283,6 +287,11 @@ class NativeSystem:
         b = np.zeros(9)
         check(library().md_get_box(self.handle, _d(b)))
         return b.reshape(3, 3)
+
+    def mesh(self):
+        m = np.zeros(3, dtype=np.int32)
+        check(library().md_get_mesh(self.handle, _i(m)))
+        return tuple(int(v) for v in m)
 
     def evaluate(self, forces=True):
         """(total, groups[6], forces[3N] or None)."""
@@ -343,6 +352,15 @@ def probe_fixed_point(values):
                                          fixed.ctypes.data_as(C.POINTER(C.c_int64)),
                                          back.ctypes.data_as(C.POINTER(C.c_float))))
     return fixed, back
+
+
+def probe_cell_order(dims):
+    """The neighbour search's cell sort order for a grid of ``dims`` cells
+    (host only)."""
+    d = np.asarray(dims, dtype=np.int32)
+    cells = np.zeros(int(np.prod(d)), dtype=np.int32)
+    check(library().md_probe_cell_order(_i(d), _i(cells)))
+    return cells
 
 
 def probe_thermostat(seed: int, step: int, n: int):

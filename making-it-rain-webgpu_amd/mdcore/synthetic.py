@@ -93,10 +93,11 @@ def _dihedral(p0, p1, p2, p3):
 def _chain(top: _Topology, rng, n_atoms: int, center, radius: float, wall: float):
     """A self-avoiding random-walk chain of exactly n_atoms (heavy + H).
 
-    Every atom, hydrogens included, is kept at least 0.25 nm from every atom
-    it is not bonded to or angled with; hydrogens take tetrahedral slots
-    around the bond to the previous heavy atom, and the next heavy atom has
-    to find a free slot, so at most two hydrogens sit on an inner atom."""
+    Heavy atoms are kept 0.25 nm from every atom they are not bonded or
+    angled to where the walk can manage it (else the roomiest of 600 tries);
+    an atom's hydrogens are placed once both its heavy neighbours are known,
+    in the sp3 slots those bonds leave (two on an inner atom, three on an
+    end), so every H-C-X angle starts near 109.5 degrees."""
     start = top.n
     heavy: List[int] = []
     hs_of = {}
@@ -135,17 +136,78 @@ def _chain(top: _Topology, rng, n_atoms: int, center, radius: float, wall: float
     prev_dir = rng.normal(size=3)
     prev_dir /= np.linalg.norm(prev_dir)
     cos_t, sin_t = math.cos(math.radians(109.5)), math.sin(math.radians(109.5))
+    half = math.radians(0.5 * 109.5)
+
+    def unit(v):
+        return v / np.linalg.norm(v)
+
+    def any_perpendicular(b):
+        e = np.cross(b, rng.normal(size=3))
+        return unit(e)
+
+    def inner_slots(c, pa, pb):
+        """The two sp3 directions an atom at c keeps free between bonds to
+        pa and pb: off the bisector, in the plane normal to the bonds."""
+        b0, b1 = unit(pa - c), unit(pb - c)
+        u = -(b0 + b1)
+        if np.linalg.norm(u) < 1e-3:
+            u = any_perpendicular(b0)
+        u = unit(u)
+        w = np.cross(b0, b1)
+        w = unit(w) if np.linalg.norm(w) > 1e-3 else unit(np.cross(u, b0))
+        return [math.cos(half) * u + math.sin(half) * w, math.cos(half) * u - math.sin(half) * w]
+
+    def slot_room(i, count, pa, pb):
+        """Clearance of the `count` roomiest hydrogen slots atom i would get
+        with heavy neighbours at pa and pb."""
+        c = top.pos[i]
+        room = sorted((nearest(c + 0.109 * s, {i, heavy[-2]}) for s in inner_slots(c, pa, pb)), reverse=True)
+        return room[count - 1]
+
+    def place_hydrogens(i, count, nbrs):
+        """`count` hydrogens on heavy atom i in sp3 slots: the two slots
+        off the bisector of an inner atom's bonds, or the three slots of
+        the 109.5-degree cone around an end atom's bond; the roomiest
+        slots win."""
+        c = top.pos[i]
+        bonds = [unit(top.pos[j] - c) for j in nbrs]
+        if len(bonds) == 2:
+            slots = inner_slots(c, top.pos[nbrs[0]], top.pos[nbrs[1]])
+        else:
+            b = bonds[0] if bonds else unit(rng.normal(size=3))
+            e1 = any_perpendicular(b)
+            e2 = np.cross(b, e1)
+            phi0 = rng.uniform(0, 2 * math.pi)
+            slots = [cos_t * b + sin_t * (math.cos(phi0 + t) * e1 + math.sin(phi0 + t) * e2)
+                     for t in (0.0, 2 * math.pi / 3, 4 * math.pi / 3)]
+        skip = {i} | set(nbrs)
+        room = [nearest(c + 0.109 * s, skip) for s in slots]
+        hs = []
+        for s in sorted(range(len(slots)), key=lambda t: -room[t])[:count]:
+            hs.append(put(c + 0.109 * slots[s], 1.008, 0.2649, 0.0657))
+            top.cons.append((i, hs[-1], 0.109))
+        hs_of[i] = hs
+
     for k in range(len(counts)):
         if k == 0:
-            p = center + rng.normal(0, 0.05, 3)
+            # the first atom: near the centre, clear of anything already built
+            p, room = None, -1.0
+            for attempt in range(400):
+                q = center + rng.normal(0, 0.05 + 0.005 * attempt, 3)
+                r = nearest(q, set())
+                if r > room:
+                    p, room = q, r
+                if r >= 0.3:
+                    break
         else:
             last = heavy[-1]
-            skip = {last} | set(hs_of[last])
+            skip = {last}
             if len(heavy) > 1:
                 skip |= {heavy[-2]} | set(hs_of[heavy[-2]])
             # ~111 degree bonds inside the sphere first; then outside it;
-            # then any bend from 90 to 150 degrees; else the roomiest try
+            # then any bend from 90 to 150 degrees; else the roomiest try.
             best, p, room = None, None, -1.0
+            inmost, inmost_d = None, float("inf")
             for attempt in range(600):
                 perp = rng.normal(size=3)
                 perp -= np.dot(perp, prev_dir) * prev_dir
@@ -155,36 +217,29 @@ def _chain(top: _Topology, rng, n_atoms: int, center, radius: float, wall: float
                 d = math.cos(theta) * prev_dir + math.sin(theta) * perp
                 q = top.pos[last] + 0.153 * d
                 outward = np.linalg.norm(q - center)
+                if outward < inmost_d:
+                    inmost, inmost_d = d, outward
                 # a hard wall keeps every periodic image of the chain apart
                 if outward > wall or (attempt < 200 and outward > radius):
                     continue
                 r = nearest(q, skip)
+                # the bond also fixes where the last atom's hydrogens go
+                if counts[k - 1] and len(heavy) > 1 and r > room:
+                    r = min(r, slot_room(last, counts[k - 1], top.pos[heavy[-2]], q) + 0.05)
                 if r > room:
                     p, best, room = q, d, r
                 if r >= 0.25:
                     break
+            if best is None:  # every try crossed the wall: turn inwards
+                best = inmost
+                p = top.pos[last] + 0.153 * best
             prev_dir = best / np.linalg.norm(best)
         i = put(p, 12.011, float(rng.uniform(0.30, 0.40)), float(rng.uniform(0.2, 0.9)))
         heavy.append(i)
-        hs = []
-        if counts[k]:
-            b = -prev_dir if k > 0 else rng.normal(size=3)
-            b = b / np.linalg.norm(b)
-            e1 = np.cross(b, rng.normal(size=3))
-            e1 /= np.linalg.norm(e1)
-            e2 = np.cross(b, e1)
-            phi0 = rng.uniform(0, 2 * math.pi)
-            skip = {i} | ({heavy[-2]} if len(heavy) > 1 else set())
-            for h in range(counts[k]):
-                for attempt in range(24):
-                    phi = phi0 + 2 * math.pi * (h + attempt / 3.0) / 3.0
-                    dvec = cos_t * b + sin_t * (math.cos(phi) * e1 + math.sin(phi) * e2)
-                    q = top.pos[i] + 0.109 * dvec
-                    if not clash(q, skip, 0.15):
-                        break
-                hs.append(put(q, 1.008, 0.2649, 0.0657))
-                top.cons.append((i, hs[-1], 0.109))
-        hs_of[i] = hs
+        # the previous atom now has both neighbours: its hydrogens go in
+        if k >= 1:
+            place_hydrogens(heavy[-2], counts[k - 1], heavy[max(0, k - 2):k - 1] + [i])
+    place_hydrogens(heavy[-1], counts[-1], heavy[-2:-1])
     # bonded terms from the built geometry
     for a, b in zip(heavy[:-1], heavy[1:]):
         top.bonds.append((a, b, float(np.linalg.norm(top.pos[b] - top.pos[a])), 259408.0))

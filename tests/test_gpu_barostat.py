@@ -1,0 +1,103 @@
+"""The Monte Carlo barostat on the GPU engine against the oracle.
+
+* after setBox with molecule-scaled positions (the mesh fixed), device forces
+  are within 2e-6 relative RMS and group energies within 1e-5 of the oracle
+  at the same box and mesh (src/gpu/loop.ts:1031-1097);
+* one Metropolis attempt with the device's potential and one with the
+  oracle's, on the same deviates, agree on the decision and on the work;
+* an NPT run attempts every 25 steps, keeps every constraint and leaves the
+  device box equal to the host's.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from mdcore import barostat as baro
+from mdcore import synthetic
+from mdcore.engine import HipEngine, RunOptions, minimise
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def npt():
+    b = synthetic.build(300, 700, 2, "cube", 77, barostat=True)
+    e = HipEngine(b.system, b.state, RunOptions(seed=4))
+    minimise(e, 200, 10.0)
+    e.setVelocitiesToTemperature(300.0, 4)
+    e.run(100)
+    yield b, e
+    e.destroy()
+
+
+def _rel_rms(a, b):
+    return float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2)))
+
+
+def test_forces_after_set_box_match_oracle(npt):
+    b, e = npt
+    x = e.positions()
+    box = np.asarray(e.boxVectors())
+    mol = baro.find_molecules(b.system)
+    y = baro.scale_molecules(x, mol, 1.02)
+    e.setBox(box * 1.02)
+    e.setPositions(y)
+    total, groups, f = e._native.evaluate(True)
+    o = O.OracleSystem(b.system)
+    ot, og, of = o.potential(y, box * 1.02, grid=e.mesh)
+    err = _rel_rms(f, of)
+    gerr = [abs(groups[k] - og[k]) / max(abs(og[k]), 1.0) for k in range(6)]
+    print("set_box: mesh", e.mesh, "force rel rms", err, "group errors", gerr)
+    e.setBox(box)
+    e.setPositions(x)
+    assert err < 2e-6
+    assert max(gerr) < 1e-5
+
+
+def test_attempt_agrees_with_oracle_attempt(npt):
+    b, e = npt
+    x = e.positions()
+    box = np.asarray(e.boxVectors())
+    mol = baro.find_molecules(b.system)
+    o = O.OracleSystem(b.system)
+    mesh = e.mesh
+
+    def gpu_potential(trial, trial_box):
+        e.setBox(trial_box)
+        e.setPositions(trial)
+        return e._native.evaluate(False)[0]
+
+    def cpu_potential(trial, trial_box):
+        return o.potential(np.asarray(trial), trial_box, grid=mesh)[0]
+
+    for draws in ([0.3, 0.5], [0.8, 0.2]):
+        sg = baro.BarostatState.initial(box)
+        sc = baro.BarostatState.initial(box)
+        ig, ic = iter(draws), iter(draws)
+        rg = baro.attempt(b.system.force("MonteCarloBarostat"), sg, box, x, gpu_potential, lambda: next(ig), mol)
+        rc = baro.attempt(b.system.force("MonteCarloBarostat"), sc, box, x, cpu_potential, lambda: next(ic), mol)
+        print("attempt: gpu work", rg.work, "oracle work", rc.work, "accepted", rg.accepted, rc.accepted)
+        assert rg.accepted == rc.accepted
+        assert abs(rg.work - rc.work) < 1e-5 * abs(rc.initialEnergy)
+        assert abs(rg.initialEnergy - rc.initialEnergy) < 1e-5 * abs(rc.initialEnergy)
+    e.setBox(box)
+    e.setPositions(x)
+
+
+def test_npt_run_attempts_on_schedule_and_keeps_constraints(npt):
+    b, e = npt
+    start_box = np.asarray(e.boxVectors())
+    before = e.barostatAttempts
+    e.run(250)
+    assert e.barostatAttempts - before == 10
+    x = e.positions().reshape(-1, 3)
+    s = b.system
+    r = np.linalg.norm(x[s.constraintP1] - x[s.constraintP2], axis=1)
+    rel = np.abs(r - s.constraintDistance) / s.constraintDistance
+    dev_box = e._native.box()
+    print("attempts", e.barostatAttempts, "accepted", e.barostatAccepted, "box", np.diag(dev_box),
+          "from", np.diag(start_box), "constraint error", rel.max())
+    np.testing.assert_allclose(dev_box, e._box, rtol=0, atol=1e-12)
+    assert rel.max() < 2e-6
+    assert np.all(np.isfinite(x))
+    assert e.barostatAccepted > 0

@@ -129,13 +129,64 @@ def test_config1_triclinic_forces_and_groups(c1):
     s.close()
 
 
+def test_boundary_frame_counts_each_pair_once(c0):
+    """A config-0 frame whose super-clusters straddle the periodic boundary,
+    with a water pair inside one cluster interacting through the boundary:
+    each such pair must be counted once."""
+    x = np.load(os.path.join(GOLDEN, "boundary-frame-config0.npy"))
+    assert x.shape[0] == 3 * c0.system.particleCount
+    o = O.OracleSystem(c0.system)
+    grid = o.default_grid(c0.system.boxVectors)
+    s = native.NativeSystem(c0.system, mesh=grid)
+    s.set_positions(x)
+    total, groups, forces = s.evaluate()
+    _, og, of = o.potential(x, c0.system.boxVectors, grid=grid)
+    print("boundary frame force rel RMS", rel_rms(forces, of), "direct", groups[3], og[3])
+    assert rel_rms(forces, of) < FORCE_RMS
+    assert abs(groups[3] - og[3]) < ENERGY_REL * abs(og[3])
+    s.close()
+
+
+def test_forces_are_invariant_under_translations(c0):
+    """Whole-system translations move every cluster across the periodic
+    boundaries in turn; the forces (and the oracle's) do not change.  On an
+    equilibrated frame: a raw generated geometry has contacts whose r^-13
+    forces turn f32 position rounding into 1e-5 relative force changes."""
+    x = np.load(os.path.join(GOLDEN, "boundary-frame-config0.npy"))
+    o = O.OracleSystem(c0.system)
+    grid = o.default_grid(c0.system.boxVectors)
+    _, og, of = o.potential(x, c0.system.boxVectors, grid=grid)
+    s = native.NativeSystem(c0.system, mesh=grid)
+    rng = np.random.default_rng(7)
+    L = np.asarray(c0.system.boxVectors)
+    worst = 0.0
+    for k in range(24):
+        t = rng.uniform(-1.0, 1.0, 3) @ L
+        s.set_positions((x.reshape(-1, 3) + t).reshape(-1))
+        _, groups, forces = s.evaluate()
+        err = rel_rms(forces, of)
+        worst = max(worst, err)
+        assert err < FORCE_RMS, (k, t, err)
+        assert abs(groups[3] - og[3]) < ENERGY_REL * abs(og[3]), (k, groups[3], og[3])
+    print("worst force rel RMS over 24 translations", worst)
+    s.close()
+
+
 def test_pair_census_matches_oracle(c0):
     o = O.OracleSystem(c0.system)
     d, _ = o.direct_space(c0.state.positions, c0.system.boxVectors, with_forces=False)
     s = native.NativeSystem(c0.system)
     s.set_positions(c0.state.positions)
     pairs, tiles = s.pair_census()
-    assert pairs == d.pairs_geometric
+    # the device tests r^2 < rc^2 in f32 on frame-relative coordinates, the
+    # oracle in f64: only pairs within rounding of the cutoff may differ
+    from scipy.spatial import cKDTree
+    x = np.mod(c0.state.positions.reshape(-1, 3), np.diag(np.asarray(c0.system.boxVectors)))
+    tree = cKDTree(x, boxsize=np.diag(np.asarray(c0.system.boxVectors)))
+    rc = c0.system.force("NonbondedForce").cutoff
+    edge = len(tree.query_pairs(rc + 2e-6)) - len(tree.query_pairs(rc - 2e-6))
+    print("census", pairs, "oracle", d.pairs_geometric, "pairs within 2e-6 nm of the cutoff", edge)
+    assert abs(pairs - d.pairs_geometric) <= edge
     assert tiles > 0
     s.close()
 

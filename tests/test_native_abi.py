@@ -50,3 +50,22 @@ def test_descriptor_packs_every_force():
     assert d.n_exceptions == s.force("NonbondedForce").exceptionCount
     assert d.n_restraints > 0 and d.has_barostat == 1 and d.cmm_frequency == 1
     assert abs(d.cutoff - 1.0) < 1e-15
+
+
+@pytest.mark.parametrize("dims", [(2, 2, 2), (8, 8, 8), (22, 22, 22), (8, 10, 14), (4, 2, 30), (20, 18, 16),
+                                  (2, 2, 64), (30, 4, 2)])
+def test_cell_walk_visits_every_cell_through_face_neighbours(dims):
+    """The cell sort order (host side of the library, no device): each cell
+    once, each step to a face neighbour -- runs of cells never jump across
+    the grid or wrap the box."""
+    cells = native.probe_cell_order(dims)
+    W, H, D = dims
+    assert np.array_equal(np.sort(cells), np.arange(W * H * D))
+    x, y, z = cells // (H * D), (cells // D) % H, cells % D
+    step = np.abs(np.diff(x)) + np.abs(np.diff(y)) + np.abs(np.diff(z))
+    assert np.all(step == 1)
+
+
+def test_cell_walk_rejects_bad_grids():
+    with pytest.raises(native.MdError):
+        native.probe_cell_order((0, 4, 4))
