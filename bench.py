@@ -1,22 +1,33 @@
-"""Benchmark: ns/day of the MD core on one MI355X, with its roofline and the
-CPU oracle timed beside it.
+"""Benchmark: aggregate ns/day of the MD core on one MI355X, batched vs plain,
+with the dominant kernel's roofline and the CPU oracle timed beside it.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ensemble|single]
 
-Workload: the 26,749-atom system (a 2,600-atom AMBER-typed chain, a 24-atom
-ligand, 8 ions and 8,039 TIP3P waters in a truncated octahedron at 1.0 g/cm^3,
-PME 1.0 nm cutoff, order 5, tolerance 5e-4, H-bond constraints + SETTLE,
-LFMiddle Langevin 300 K / 1 ps^-1, dt 2 fs, fp32 arithmetic with Q32.32
-int64 accumulation), built synthetically with a fixed seed, minimised, given
-Maxwell-Boltzmann velocities and warmed up before the timed region.  The
-plain single-system path runs strides of 50 steps captured as HIP graphs.
+Workload "ensemble" (the default; the metric's configuration): 64 independent
+solvated chains whose atom counts are log-uniform on 2,000-30,000 (647,236
+atoms in all), cubic and truncated-octahedral cells alternating, per-replica
+seeds and temperatures 290-320 K, PME 1.0 nm cutoff, order 5, tolerance 5e-4,
+H-bond constraints + SETTLE, LFMiddle Langevin 1 ps^-1, dt 2 fs, fp32
+arithmetic with Q32.32 int64 accumulation.  Built synthetically from a fixed
+seed, minimised, given Maxwell-Boltzmann velocities and warmed up.  Then the
+same replicas are timed two ways for K steps each:
 
-Prints one JSON line (rank 0).  Progress goes to stderr.
+* plain: one after another, each through its own single-system engine
+  (strides of 50 steps captured as HIP graphs);
+* batched: all 64 together, one kernel launch per phase for every replica,
+  strides captured as HIP graphs.
+
+`value` is the batched aggregate ns/day: (sum over replicas of K * dt) / wall
+time.  The forces of sampled replicas are checked against the f64 oracle
+after the batched run (the metric's fixed force error, 2e-6 relative RMS).
+
+Workload "single": the 26,749-atom system alone (truncated octahedron), the
+plain path's single-system throughput.
+
+Prints one JSON line.  Progress goes to stderr.
 """
 import argparse
-import ctypes
 import json
-import math
 import os
 import sys
 import time
@@ -30,6 +41,8 @@ import numpy as np  # noqa: E402
 FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
 PAIR_FLOPS = 76            # algorithmic FP32 flops per in-cutoff pair
+DT_PS = 0.002
+METRIC = "aggregate ns/day at 2 fs (batched+learned vs plain, same GPU, fixed force error)"
 
 
 def log(msg):
@@ -37,106 +50,141 @@ def log(msg):
 
 
 def device_synchronize():
+    import ctypes
     hip = ctypes.CDLL("libamdhip64.so")
     rc = hip.hipDeviceSynchronize()
     if rc != 0:
         raise RuntimeError(f"hipDeviceSynchronize failed: {rc}")
 
 
-def cpu_baseline(built, positions, velocities, budget_s):
-    """The oracle (f64 C restatement of the reference's CPU path), one thread,
-    LFMiddle steps on the same system and state until ~budget_s seconds."""
+def _build_one(plan):
+    from mdcore import synthetic
+    b = synthetic.build(plan["n_chain"], plan["n_waters"], plan["n_ions"], plan["shape"], plan["seed"])
+    return synthetic.Replica(b, plan["seed"], plan["temperature"])
+
+
+def build_ensemble(count, workers):
+    """The ensemble's replicas, built in spawned worker processes (no GPU is
+    touched before they finish)."""
+    from mdcore import synthetic
+    plan = synthetic.ensemble_plan(count)
+    if workers <= 1:
+        return [_build_one(p) for p in plan]
+    import multiprocessing as mp
+    with mp.get_context("spawn").Pool(workers) as pool:
+        return pool.map(_build_one, plan, chunksize=1)
+
+
+def oracle_rate(system, positions, velocities, budget_s, temperature=300.0, seed=1):
+    """Atom-steps/s of the f64 C oracle (the reference CPU path restated) on
+    one host core: LFMiddle steps with SETTLE/SHAKE and the thermostat."""
     import oracle as O
-    o = O.OracleSystem(built.system)
-    grid = o.default_grid(built.system.boxVectors)
-    pos = positions.copy()
-    vel = velocities.copy()
+    o = O.OracleSystem(system)
+    grid = o.default_grid(system.boxVectors)
+    pos, vel = positions.copy(), velocities.copy()
     steps = 0
     t0 = time.perf_counter()
     while True:
-        o.run_lfmiddle(pos, vel, built.system.boxVectors, 1, 0.002, 300.0, 1.0, 1e-6, 1, steps, grid)
+        o.run_lfmiddle(pos, vel, system.boxVectors, 1, DT_PS, temperature, 1.0, 1e-6, seed, steps, grid)
         steps += 1
-        if time.perf_counter() - t0 >= budget_s or steps >= 50:
+        if time.perf_counter() - t0 >= budget_s or steps >= 200:
             break
     el = time.perf_counter() - t0
-    ns_day = steps * 0.002e-3 / el * 86400.0
-    return {"value": ns_day, "unit": "ns/day", "cores": 1, "kind": "port",
-            "sample": f"{steps} LFMiddle steps (forces, SETTLE/SHAKE, thermostat) of the same "
-                      f"{built.system.particleCount}-atom system by the f64 C oracle on one host core, "
-                      f"{el:.1f} s", "seconds": el, "steps": steps}
+    return system.particleCount * steps / el, steps, el
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5000)
-    ap.add_argument("--warmup", type=int, default=500)
-    ap.add_argument("--profile-steps", type=int, default=50)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--minimise-iterations", type=int, default=300)
-    ap.add_argument("--max-batch", type=int, default=0,
-                    help="steps per captured HIP graph (0: 50; 1: launch every step directly)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU: this build is single-GPU (asked for --gpus {args.gpus}). "
-              "Independent replicas would shard across GPUs with no communication between them.",
-              file=sys.stderr)
-        sys.exit(2)
+def force_error(engine, system):
+    import oracle as O
+    x = engine.positions()
+    _, _, f = engine._native.evaluate(True)
+    o = O.OracleSystem(system)
+    _, _, of = o.potential(x, system.boxVectors, grid=engine.mesh)
+    return float(np.sqrt(np.sum((f - of) ** 2) / np.sum(of ** 2)))
 
-    from mdcore import synthetic
-    from mdcore.engine import HipEngine, RunOptions
+
+def pair_roofline(pairs, kernel_ms):
+    achieved = pairs * PAIR_FLOPS / (kernel_ms * 1e-3) / 1e12 if kernel_ms > 0 else 0.0
+    return {
+        "kernel": "k_nb_pairs (LJ + erfc-Ewald cluster-pair kernel)",
+        "bound": "valu",
+        "achieved": achieved,
+        "peak": FP32_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": achieved / FP32_PEAK_TFLOPS,
+        "traffic": None,
+        "algorithmic": f"{PAIR_FLOPS} flop x {int(pairs)} in-cutoff pairs per launch",
+        "kernel_ms": kernel_ms,
+    }
+
+
+def run_ensemble(args):
+    from mdcore.engine import HipBatch, HipEngine, RunOptions, minimise
 
     t = time.perf_counter()
-    built = synthetic.config(1)
-    n = built.system.particleCount
-    log(f"built {n} atoms ({built.description['shape']}, {built.description['width_nm']:.3f} nm) "
-        f"in {time.perf_counter() - t:.1f} s")
-    engine = HipEngine(built.system, built.state, RunOptions(timeStep=0.002, temperature=300.0, friction=1.0,
-                                                            constraintTolerance=1e-6, seed=2026,
-                                                            maxBatch=args.max_batch))
+    workers = max(1, min(16, os.cpu_count() or 1))
+    reps = build_ensemble(args.replicas, workers)
+    atoms = [r.built.system.particleCount for r in reps]
+    total_atoms = int(sum(atoms))
+    log(f"built {len(reps)} replicas, {total_atoms} atoms ({min(atoms)}-{max(atoms)}) in "
+        f"{time.perf_counter() - t:.1f} s with {workers} workers")
+
     t = time.perf_counter()
-    from mdcore.engine import minimise
-    res = minimise(engine, args.minimise_iterations, 10.0)
-    log(f"minimised to {res.energy:.1f} kJ/mol in {res.iterations} iterations, {time.perf_counter() - t:.1f} s")
-    engine.setVelocitiesToTemperature(300.0, 2026)
-    t = time.perf_counter()
-    engine.run(args.warmup)
+    engines = []
+    for r in reps:
+        e = HipEngine(r.built.system, r.built.state,
+                      RunOptions(timeStep=DT_PS, temperature=r.temperature, friction=1.0, constraintTolerance=1e-6,
+                                 seed=r.seed, maxBatch=args.max_batch))
+        minimise(e, args.minimise_iterations, 10.0)
+        e.setVelocitiesToTemperature(r.temperature, r.seed)
+        e.run(args.warmup)
+        engines.append(e)
     device_synchronize()
-    rep = engine.report()
-    log(f"warmed up {args.warmup} steps in {time.perf_counter() - t:.1f} s: T = {rep.temperature:.1f} K, "
-        f"E = {rep.totalEnergy:.1f}")
+    log(f"minimised, thermalised and warmed up {len(engines)} replicas in {time.perf_counter() - t:.1f} s")
 
-    # ---- timed region ---------------------------------------------------
+    # ---- plain: replicas one after another through the single-system path --
+    plain_steps = args.plain_steps
+    for e in engines:  # capture each replica's stride graph outside the timed region
+        e.run(50)
     device_synchronize()
     t0 = time.perf_counter()
-    engine.run(args.steps)
+    for e in engines:
+        e.run(plain_steps)
     device_synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = elapsed / args.steps * 1000.0
-    ns_day = args.steps * 0.002e-3 / elapsed * 86400.0
-    rep = engine.report()
-    log(f"timed {args.steps} steps: {ms_per_step * 1000:.1f} us/step, {ns_day:.1f} ns/day, "
-        f"T = {rep.temperature:.1f} K")
+    plain_s = time.perf_counter() - t0
+    plain_nsday = len(engines) * plain_steps * DT_PS * 1e-3 / plain_s * 86400.0
+    log(f"plain: {len(engines)} x {plain_steps} steps in {plain_s:.2f} s = {plain_nsday:.1f} ns/day aggregate")
 
-    # ---- kernel roofline: per-phase HIP-event timing on the engine's stream --
-    native = engine._native
-    pairs, entries = native.pair_census()
-    phases = native.profile_steps(args.profile_steps)
-    pairs2, _ = native.pair_census()
-    pairs_avg = 0.5 * (pairs + pairs2)
-    t_pairs = phases["pairs"] * 1e-3
-    achieved = pairs_avg * PAIR_FLOPS / t_pairs / 1e12 if t_pairs > 0 else 0.0
-    step_ms = sum(phases.values())
-    log("phases (ms): " + ", ".join(f"{k} {v:.4f}" for k, v in phases.items()))
-    # the fused integrator is HBM-bound: 124 algorithmic bytes per atom-step
-    integ_bytes = 124.0 * n
-    integ_gbs = integ_bytes / (phases["integrate"] * 1e-3) / 1e9 if phases["integrate"] > 0 else 0.0
-    flags = engine.verletFlags()
+    # ---- batched ---------------------------------------------------------------
+    batch = HipBatch(engines)
+    batch.run(max(args.warmup, 50))
+    device_synchronize()
+    t0 = time.perf_counter()
+    batch.run(args.steps)
+    device_synchronize()
+    batch_s = time.perf_counter() - t0
+    batch_nsday = len(engines) * args.steps * DT_PS * 1e-3 / batch_s * 86400.0
+    ms_per_step = batch_s / args.steps * 1e3
+    log(f"batched: {args.steps} steps of all replicas in {batch_s:.2f} s = {ms_per_step:.3f} ms/step, "
+        f"{batch_nsday:.1f} ns/day aggregate, {batch_nsday / plain_nsday:.2f}x plain")
+
+    # ---- fixed force error: sampled replicas against the oracle ---------------
+    order = np.argsort(atoms)
+    sample = [int(order[0]), int(order[len(order) // 2])]
+    errors = {int(atoms[k]): force_error(engines[k], reps[k].built.system) for k in sample}
+    log(f"force rel RMS vs oracle after the batched run: {errors}")
+
+    # ---- roofline of the dominant kernel, measured inside the batched step -----
+    phases = batch.profileStep(args.profile_steps)
+    pairs = 0
+    for e in engines:
+        p, _ = e._native.pair_census()
+        pairs += p
+    log("batched phases (ms): " + ", ".join(f"{k} {v:.4f}" for k, v in phases.items()))
+    integ_gbs = 124.0 * total_atoms / (phases["integrate"] * 1e-3) / 1e9 if phases["integrate"] > 0 else 0.0
 
     result = {
-        "metric": "aggregate ns/day at 2 fs (batched+learned vs plain, same GPU, fixed force error)",
-        "value": ns_day,
+        "metric": METRIC,
+        "value": batch_nsday,
         "unit": "ns/day",
         "n_gpus": 1,
         "steps": args.steps,
@@ -145,40 +193,117 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 arithmetic, int64 Q32.32 accumulation",
-        "data": "synthetic (seeded generator: chain + ligand + TIP3P + ions), minimised and thermalised",
+        "dtype": "fp32",
+        "data": "synthetic (seeded generator: chains + TIP3P + ions), minimised and thermalised",
         "config": {
-            "workload": "plain single-system path: 26,749 atoms, truncated octahedron "
-                        f"{built.description['width_nm']:.3f} nm, PME 1.0 nm cutoff, LFMiddle 300 K, dt 2 fs, "
-                        "H-bond SHAKE + SETTLE, 50-step HIP-graph strides",
-            "atoms": n,
-            "atom_steps_per_s": n * args.steps / elapsed,
-            "in_cutoff_pairs": int(pairs_avg),
-            "list_entries": int(entries),
-            "list_rebuilds": flags["rebuilds"],
+            "workload": f"batched ensemble: {len(engines)} replicas, {total_atoms} atoms (log-uniform "
+                        f"{min(atoms)}-{max(atoms)}), cubic/octahedral cells, 290-320 K, PME 1.0 nm, LFMiddle, "
+                        "dt 2 fs, SHAKE + SETTLE; one launch per phase for all replicas, 50-step HIP-graph strides",
+            "replicas": len(engines),
+            "atoms": total_atoms,
+            "plain_ns_day": plain_nsday,
+            "plain_seconds": plain_s,
+            "plain_steps_per_replica": plain_steps,
+            "batched_over_plain": batch_nsday / plain_nsday,
+            "atom_steps_per_s": total_atoms * args.steps / batch_s,
+            "learned_warm_start": "not in this measurement",
+            "force_rel_rms_vs_oracle": errors,
+            "in_cutoff_pairs": int(pairs),
+            "arithmetic": "fp32 with Q32.32 int64 accumulation",
             "parallelism": "single GPU",
         },
-        "roofline": {
-            "kernel": "k_nb_pairs (LJ + erfc-Ewald pair kernel)",
-            "bound": "valu",
-            "achieved": achieved,
-            "peak": FP32_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": None,
-            "algorithmic": f"{PAIR_FLOPS} flop x {int(pairs_avg)} in-cutoff pairs per launch",
-            "kernel_ms": phases["pairs"],
-        },
+        "roofline": pair_roofline(pairs, phases["pairs"]),
         "phases_ms": phases,
-        "profiled_step_ms": step_ms,
         "integrator_roofline": {"bound": "hbm", "achieved": integ_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": integ_gbs / HBM_PEAK_GBS, "bytes_per_atom": 124},
     }
     if not args.no_cpu_baseline:
-        log(f"timing the CPU oracle for ~{args.cpu_seconds:.0f} s")
-        result["cpu_baseline"] = cpu_baseline(built, engine.positions(), engine.velocities(), args.cpu_seconds)
-        log(f"cpu baseline {result['cpu_baseline']['value']:.4f} ns/day")
+        k = int(order[0])
+        log(f"timing the CPU oracle on the {atoms[k]}-atom replica for ~{args.cpu_seconds:.0f} s")
+        rate, steps, el = oracle_rate(reps[k].built.system, engines[k].positions(), engines[k].velocities(),
+                                      args.cpu_seconds, reps[k].temperature, reps[k].seed)
+        # the whole ensemble, one replica after another, at that atom-step rate
+        cpu_nsday = len(engines) * DT_PS * 1e-3 * rate / total_atoms * 86400.0
+        result["cpu_baseline"] = {
+            "value": cpu_nsday, "unit": "ns/day", "cores": 1, "kind": "port",
+            "sample": f"{steps} LFMiddle steps of the {atoms[k]}-atom replica by the f64 C oracle on one host "
+                      f"core ({el:.1f} s, {rate:.0f} atom-steps/s), scaled to the whole ensemble run one "
+                      "replica after another at that atom-step rate",
+            "atom_steps_per_s": rate}
+        log(f"cpu baseline {cpu_nsday:.5f} ns/day aggregate")
+    batch.destroy()
+    for e in engines:
+        e.destroy()
+    return result
+
+
+def run_single(args):
+    from mdcore import synthetic
+    from mdcore.engine import HipEngine, RunOptions, minimise
+
+    built = synthetic.config(1)
+    n = built.system.particleCount
+    engine = HipEngine(built.system, built.state, RunOptions(timeStep=DT_PS, temperature=300.0, friction=1.0,
+                                                            constraintTolerance=1e-6, seed=2026,
+                                                            maxBatch=args.max_batch))
+    res = minimise(engine, args.minimise_iterations, 10.0)
+    log(f"minimised to {res.energy:.1f} kJ/mol in {res.iterations} iterations")
+    engine.setVelocitiesToTemperature(300.0, 2026)
+    engine.run(args.warmup)
+    device_synchronize()
+    t0 = time.perf_counter()
+    engine.run(args.steps)
+    device_synchronize()
+    elapsed = time.perf_counter() - t0
+    ns_day = args.steps * DT_PS * 1e-3 / elapsed * 86400.0
+    log(f"timed {args.steps} steps: {elapsed / args.steps * 1e6:.1f} us/step, {ns_day:.1f} ns/day")
+    pairs, _ = engine._native.pair_census()
+    phases = engine.profileStep(args.profile_steps)
+    log("phases (ms): " + ", ".join(f"{k} {v:.4f}" for k, v in phases.items()))
+    err = force_error(engine, built.system)
+    result = {
+        "metric": METRIC, "value": ns_day, "unit": "ns/day", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded generator: chain + ligand + TIP3P + ions), minimised and thermalised",
+        "config": {"workload": f"plain single-system path: {n} atoms, truncated octahedron, PME 1.0 nm, LFMiddle "
+                               "300 K, dt 2 fs, SHAKE + SETTLE, 50-step HIP-graph strides",
+                   "atoms": n, "atom_steps_per_s": n * args.steps / elapsed, "in_cutoff_pairs": int(pairs),
+                   "force_rel_rms_vs_oracle": err, "parallelism": "single GPU"},
+        "roofline": pair_roofline(pairs, phases["pairs"]),
+        "phases_ms": phases,
+    }
+    if not args.no_cpu_baseline:
+        rate, steps, el = oracle_rate(built.system, engine.positions(), engine.velocities(), args.cpu_seconds)
+        result["cpu_baseline"] = {"value": DT_PS * 1e-3 * rate / n * 86400.0, "unit": "ns/day", "cores": 1,
+                                  "kind": "port",
+                                  "sample": f"{steps} LFMiddle steps of the same system by the f64 C oracle on one "
+                                            f"host core, {el:.1f} s"}
     engine.destroy()
+    return result
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", choices=("ensemble", "single"), default="ensemble")
+    ap.add_argument("--replicas", type=int, default=64)
+    ap.add_argument("--plain-steps", type=int, default=200)
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--minimise-iterations", type=int, default=60)
+    ap.add_argument("--max-batch", type=int, default=0,
+                    help="steps per captured HIP graph (0: 50; 1: launch every step directly)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build is single-GPU (asked for --gpus {args.gpus}). "
+              "Independent replicas would shard across GPUs with no communication between them.",
+              file=sys.stderr)
+        sys.exit(2)
+    result = run_ensemble(args) if args.workload == "ensemble" else run_single(args)
     print(json.dumps(result), flush=True)
 
 

@@ -163,6 +163,59 @@ md_err md_run(md_system *sys, int64_t steps);
 md_err md_get_step(md_system *sys, int64_t *step);
 md_err md_set_step(md_system *sys, int64_t step);
 
+/* Batched replicas.  No counterpart upstream: StepLoop (src/gpu/loop.ts:208)
+ * runs one system per device, and this attaches to its per-system state and
+ * step ring (loop.ts:171, 434-445, 715-751).  A batch advances several
+ * independent systems of any sizes together: each phase of a step is ONE
+ * kernel launch covering every replica (the pair kernel's waves draw list
+ * segments of all replicas from one counter), and strides of steps are
+ * captured as HIP graphs.  Each replica keeps its own state, options
+ * (temperature, seed, ...) and every per-system call above; the batch borrows
+ * the systems, which must all use the same device and outlive it.  Calls on
+ * a batch and on its systems must not overlap. */
+typedef struct md_batch md_batch;
+md_err md_batch_create(md_system *const *systems, int32_t count, md_batch **out);
+void md_batch_destroy(md_batch *batch);
+/* advance every replica by `steps` steps */
+md_err md_batch_run(md_batch *batch, int64_t steps);
+/* md_profile_steps for the whole batch: per-phase milliseconds of one step
+ * (launched directly, no graph), averaged over `steps` steps */
+md_err md_batch_profile_steps(md_batch *batch, int32_t steps, double *ms);
+
+/* Learned SHAKE warm start.  No counterpart upstream: it attaches where
+ * `shake_cluster` starts (src/gpu/kernels/shake.wgsl:130-146), before the
+ * oracle's multiplier accumulation (src/reference/constraints.ts:147-184).  A
+ * small network (32 -> 64 -> 64 -> 4, tanh; BF16 matrix-core operands, FP32
+ * accumulation) predicts each H-bond cluster's (up to 4 constraints) converged
+ * multipliers from rotation-invariant geometry; SHAKE applies the prediction
+ * as its first update and then iterates to the same tolerance as always, so
+ * the constraints are satisfied exactly as without it -- only passes are
+ * saved.  The network is trained on the device, on samples the engine's own
+ * SHAKE produces (md_warmstart_record), with Adam on mean-squared error.
+ * Parameters (md_warmstart_param_count floats): W1[32][64] b1[64] W2[64][64]
+ * b2[64] W3[64][16] b3[16], outputs 0..3 used. */
+typedef struct md_warmstart md_warmstart;
+md_err md_warmstart_create(int32_t device, uint32_t seed, int64_t sample_capacity, md_warmstart **out);
+void md_warmstart_destroy(md_warmstart *ws);
+int32_t md_warmstart_param_count(void);
+/* advance `sys` by up to `steps` steps (as many as the sample store holds),
+ * recording each cluster's inputs and converged multipliers */
+md_err md_warmstart_record(md_warmstart *ws, md_system *sys, int64_t steps);
+md_err md_warmstart_sample_count(md_warmstart *ws, int64_t *count);
+/* `iterations` Adam steps on minibatches of `batch` recorded samples; *loss
+ * (may be NULL) receives the last minibatch's loss */
+md_err md_warmstart_train(md_warmstart *ws, int32_t iterations, int32_t batch, double learning_rate, double *loss);
+/* start SHAKE from the network's prediction in `sys` (NULL: plain SHAKE) */
+md_err md_warmstart_attach(md_system *sys, md_warmstart *ws);
+/* read (get) and/or replace (set) the FP32 parameters */
+md_err md_warmstart_params(md_warmstart *ws, float *get, const float *set);
+/* the network on n given samples (x[n][32], t[n][4]): outputs y[n][16] (may be
+ * NULL), the loss gradient (may be NULL) and the loss, without an update */
+md_err md_warmstart_evaluate(md_warmstart *ws, const float *x, const float *t, int32_t n, float *y, float *grad,
+                             double *loss);
+/* SHAKE update passes and cluster-steps so far, all modes */
+md_err md_shake_counts(md_system *sys, int64_t *passes, int64_t *clusters);
+
 /* constrainPositions(system, x, x, tol) and constrainVelocities(system, x, v,
  * tol) (src/reference/constraints.ts:254-291) applied on the device to the
  * current positions or velocities: SETTLE for rigid waters, Jacobi SHAKE or

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "warmstart.hpp"
 
 static thread_local std::string g_last_error;
 
@@ -91,6 +92,82 @@ static int64_t fixed_checked(double x, const char *what) {
   return md_double_to_fixed(x);
 }
 
+// ---- launch groups -----------------------------------------------------------
+
+md_launch md_make_launch(md_group *g, const std::vector<int> &blocks) {
+  md_launch L;
+  std::vector<int> start(blocks.size() + 1, 0);
+  for (size_t r = 0; r < blocks.size(); ++r) start[r + 1] = start[r] + blocks[r];
+  L.total = start.back();
+  L.map.start = md_group_upload(g, start);
+  L.map.R = (int)blocks.size();
+  return L;
+}
+
+static void group_refresh(md_group *g) {
+  MD_HIP(hipSetDevice(g->device));
+  if (g->nb) md_nb_group_refresh(g);
+  md_bonded_group_refresh(g);
+  if (g->pme) md_pme_group_refresh(g);
+  md_int_group_refresh(g);
+}
+
+static void group_release(md_group *g) {
+  if (!g) return;
+  hipSetDevice(g->device);
+  if (g->stream) hipStreamSynchronize(g->stream);
+  if (g->stride_exec) hipGraphExecDestroy(g->stride_exec);
+  if (g->stride_graph) hipGraphDestroy(g->stride_graph);
+  if (g->nb) md_nb_group_free(g);
+  if (g->bonded) md_bonded_group_free(g);
+  if (g->pme) md_pme_group_free(g);
+  if (g->integ) md_int_group_free(g);
+  for (void *p : g->allocations) hipFree(p);
+  if (g->own_stream && g->stream) hipStreamDestroy(g->stream);
+  for (md_system *s : g->reps) {
+    auto &v = s->groups;
+    v.erase(std::remove(v.begin(), v.end(), g), v.end());
+  }
+  if (g->owner) *g->owner = nullptr;
+  delete g;
+}
+
+// a launch group over `reps` (all on one device), on `stream` or its own
+static md_group *group_create(const std::vector<md_system *> &reps, hipStream_t stream) {
+  md_group *g = new md_group();
+  try {
+    g->device = reps[0]->device;
+    g->reps = reps;
+    g->max_batch = reps[0]->opt.max_batch;
+    MD_HIP(hipSetDevice(g->device));
+    if (stream) {
+      g->stream = stream;
+    } else {
+      MD_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+      g->own_stream = true;
+    }
+    bool any_nb = false, all_nb = true;
+    for (md_system *s : reps) {
+      any_nb |= s->has_nonbonded;
+      all_nb &= s->has_nonbonded;
+    }
+    if (any_nb && !all_nb)
+      throw md_error(MD_ERR_INVALID_ARGUMENT, "a batch mixes systems with and without a NonbondedForce");
+    if (any_nb) {
+      md_nb_group_setup(g);
+      md_pme_group_setup(g);
+    }
+    md_bonded_group_setup(g);
+    md_int_group_setup(g);
+    MD_HIP(hipStreamSynchronize(g->stream));
+    for (md_system *s : reps) s->groups.push_back(g);
+  } catch (...) {
+    group_release(g);
+    throw;
+  }
+  return g;
+}
+
 static void apply_box(md_system *s, const double *in) {
   double b[9];
   reduce_box(in, b);
@@ -120,13 +197,9 @@ static void apply_box(md_system *s, const double *in) {
     d.inv[k] = (float)m[k];
   }
   d.rectangular = bx == 0 && cx == 0 && cy == 0;
-  // any captured stride bakes the box into its kernel arguments
-  if (s->stride_exec) {
-    hipGraphExecDestroy(s->stride_exec);
-    hipGraphDestroy(s->stride_graph);
-    s->stride_exec = nullptr;
-    s->stride_graph = nullptr;
-  }
+  // the kernels read the box from their groups' tables: refresh them (a
+  // captured stride keeps pointing at the tables, so it stays valid)
+  for (md_group *g : s->groups) group_refresh(g);
   s->need_rebuild = true;
 }
 
@@ -473,14 +546,14 @@ static void create(md_system *s, const md_system_desc *d, const md_run_options *
     md_pme_setup(s);
   }
   MD_HIP(hipStreamSynchronize(s->stream));
+  s->self = group_create({s}, s->stream);
 }
 
 static void release(md_system *s) {
   if (!s) return;
   hipSetDevice(s->device);
   if (s->stream) hipStreamSynchronize(s->stream);
-  if (s->stride_exec) hipGraphExecDestroy(s->stride_exec);
-  if (s->stride_graph) hipGraphDestroy(s->stride_graph);
+  while (!s->groups.empty()) group_release(s->groups.back());
   if (s->nb_ext) md_nb_free(s);
   if (s->pme_ext) md_pme_free(s);
   for (void *p : s->allocations) hipFree(p);
@@ -490,33 +563,35 @@ static void release(md_system *s) {
 
 // ---- steps and evaluations -------------------------------------------------
 
-void md_mark(md_system *s, int phase) {
-  if (!s->profiling) return;
+void md_mark(md_group *g, int phase) {
+  if (!g->profiling) return;
   hipEvent_t e;
   MD_HIP(hipEventCreate(&e));
-  MD_HIP(hipEventRecord(e, s->stream));
-  s->prof_events.push_back(e);
-  s->prof_phase.push_back(phase);
+  MD_HIP(hipEventRecord(e, g->stream));
+  g->prof_events.push_back(e);
+  g->prof_phase.push_back(phase);
 }
 
-static void launch_step(md_system *s, bool force_rebuild) {
-  md_mark(s, MD_PHASE_PROLOGUE);
-  md_step_prologue(s, true, force_rebuild);
-  if (s->has_nonbonded) {
-    md_mark(s, MD_PHASE_BUILD);
-    md_nb_build(s, false);
-    md_mark(s, MD_PHASE_PACK);
-    md_nb_pairs(s, false, false);
+// one step of every replica in the group
+static void launch_step(md_group *g) {
+  const bool nb = g->nb != nullptr;
+  md_mark(g, MD_PHASE_PROLOGUE);
+  md_step_prologue(g, true);
+  if (nb) {
+    md_mark(g, MD_PHASE_BUILD);
+    md_nb_build(g);
+    md_mark(g, MD_PHASE_PACK);
+    md_nb_pairs(g, false);
   }
-  md_mark(s, MD_PHASE_BONDED);
-  md_bonded(s, false);
-  if (s->has_nonbonded) {
-    md_mark(s, MD_PHASE_SPREAD);
-    md_pme(s, false);
+  md_mark(g, MD_PHASE_BONDED);
+  md_bonded(g, false);
+  if (nb) {
+    md_mark(g, MD_PHASE_SPREAD);
+    md_pme(g, false);
   }
-  md_mark(s, MD_PHASE_INTEGRATE);
-  md_integrate(s);
-  md_mark(s, -1);
+  md_mark(g, MD_PHASE_INTEGRATE);
+  md_integrate(g);
+  md_mark(g, -1);
 }
 
 static void check_control(md_system *s) {
@@ -532,6 +607,23 @@ static void check_control(md_system *s) {
                    "density); recreate the system from the current state");
 }
 
+// host state of each replica onto the device before its first step: the
+// step counter, a pending rebuild, the momentum of new velocities
+static void group_prepare(md_group *g) {
+  bool momentum = false;
+  for (md_system *s : g->reps) {
+    MD_HIP(hipMemcpyAsync(&s->ctl->step_next, &s->steps_done, sizeof(int64_t), hipMemcpyHostToDevice, g->stream));
+    if (s->need_rebuild) {
+      static const uint32_t one = 1;
+      MD_HIP(hipMemcpyAsync(&s->ctl->force_pending, &one, sizeof one, hipMemcpyHostToDevice, g->stream));
+      s->need_rebuild = false;
+    }
+    momentum |= s->momentum_dirty;
+    s->momentum_dirty = false;
+  }
+  if (momentum) md_momentum(g);
+}
+
 static void fault_tick(md_system *s, long count) {
   if (s->fault_after < 0) return;
   if (s->fault_after < count) {
@@ -541,53 +633,57 @@ static void fault_tick(md_system *s, long count) {
   s->fault_after -= count;
 }
 
+static void run_group(md_group *g, int64_t steps) {
+  if (steps <= 0) return;
+  MD_HIP(hipSetDevice(g->device));
+  group_prepare(g);
+  md_zero_forces(g, false);
+  int64_t left = steps;
+  const int stride = g->max_batch > 0 ? g->max_batch : STRIDE_STEPS;
+  if (g->graph_stale && g->stride_exec) {
+    MD_HIP(hipGraphExecDestroy(g->stride_exec));
+    MD_HIP(hipGraphDestroy(g->stride_graph));
+    g->stride_exec = nullptr;
+    g->stride_graph = nullptr;
+  }
+  g->graph_stale = false;
+  if (stride > 1 && left >= stride && !g->stride_exec) {
+    MD_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < stride; ++k) launch_step(g);
+    MD_HIP(hipStreamEndCapture(g->stream, &g->stride_graph));
+    MD_HIP(hipGraphInstantiate(&g->stride_exec, g->stride_graph, nullptr, nullptr, 0));
+    g->stride_steps = stride;
+  }
+  while (g->stride_exec && left >= g->stride_steps) {
+    MD_HIP(hipGraphLaunch(g->stride_exec, g->stream));
+    left -= g->stride_steps;
+  }
+  for (; left > 0; --left) launch_step(g);
+  MD_HIP(hipStreamSynchronize(g->stream));
+  for (md_system *s : g->reps) s->steps_done += steps;
+  for (md_system *s : g->reps) check_control(s);
+}
+
 static void run_steps(md_system *s, int64_t steps) {
   if (steps <= 0) return;
-  MD_HIP(hipSetDevice(s->device));
   fault_tick(s, (long)steps);
-  MD_HIP(hipMemcpyAsync(&s->ctl->step_next, &s->steps_done, sizeof(int64_t), hipMemcpyHostToDevice, s->stream));
-  if (s->momentum_dirty) {
-    md_momentum(s);
-    s->momentum_dirty = false;
-  }
-  md_zero_forces(s, false);
-  int64_t left = steps;
-  if (s->need_rebuild) {
-    launch_step(s, true);
-    s->need_rebuild = false;
-    --left;
-  }
-  const int stride = s->opt.max_batch > 0 ? s->opt.max_batch : STRIDE_STEPS;
-  if (stride > 1 && left >= stride && !s->stride_exec) {
-    MD_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < stride; ++k) launch_step(s, false);
-    MD_HIP(hipStreamEndCapture(s->stream, &s->stride_graph));
-    MD_HIP(hipGraphInstantiate(&s->stride_exec, s->stride_graph, nullptr, nullptr, 0));
-    s->stride_steps = stride;
-  }
-  while (s->stride_exec && left >= s->stride_steps) {
-    MD_HIP(hipGraphLaunch(s->stride_exec, s->stream));
-    left -= s->stride_steps;
-  }
-  for (; left > 0; --left) launch_step(s, false);
-  MD_HIP(hipStreamSynchronize(s->stream));
-  s->steps_done += steps;
-  check_control(s);
+  run_group(s->self, steps);
 }
 
 static void evaluate(md_system *s, double *forces, double *groups, double *total, bool census) {
   MD_HIP(hipSetDevice(s->device));
   fault_tick(s, 1);
-  md_zero_forces(s, true);
+  md_group *grp = s->self;
+  group_prepare(grp);
+  md_zero_forces(grp, true);
   if (census) MD_HIP(hipMemsetAsync(&s->ctl->pair_census, 0, sizeof(int64_t), s->stream));
-  md_step_prologue(s, false, s->need_rebuild);
-  s->need_rebuild = false;
+  md_step_prologue(grp, false);
   if (s->has_nonbonded) {
-    md_nb_build(s, false);
-    md_nb_pairs(s, true, census);
+    md_nb_build(grp);
+    md_nb_pairs(grp, true);
   }
-  md_bonded(s, true);
-  if (s->has_nonbonded) md_pme(s, true);
+  md_bonded(grp, true);
+  if (s->has_nonbonded) md_pme(grp, true);
   int64_t e[MD_E_SLOTS];
   MD_HIP(hipMemcpyAsync(e, s->energy, sizeof e, hipMemcpyDeviceToHost, s->stream));
   std::vector<int64_t> f;
@@ -648,6 +744,42 @@ __global__ void k_probe_thermostat(uint32_t seed, uint32_t step, int n, uint32_t
   normals[3 * p + 1] = r1 * sinf(t1);
   normals[3 * p + 2] = r2 * cosf(t2);
 }
+
+// per-phase device time of `steps` directly launched steps of a group, from
+// events recorded between the phases (averaged per step)
+static void profile_group(md_group *g, int steps, double *ms) {
+  MD_HIP(hipSetDevice(g->device));
+  for (int k = 0; k < MD_PHASE_COUNT; ++k) ms[k] = 0.0;
+  group_prepare(g);
+  md_zero_forces(g, false);
+  for (int r = 0; r < steps; ++r) {
+    g->profiling = true;
+    try {
+      launch_step(g);
+    } catch (...) {
+      g->profiling = false;
+      throw;
+    }
+    g->profiling = false;
+    MD_HIP(hipStreamSynchronize(g->stream));
+    for (size_t k = 0; k + 1 < g->prof_events.size(); ++k) {
+      float t = 0.0f;
+      MD_HIP(hipEventElapsedTime(&t, g->prof_events[k], g->prof_events[k + 1]));
+      int ph = g->prof_phase[k];
+      if (ph >= 0 && ph < MD_PHASE_COUNT) ms[ph] += t;
+    }
+    for (hipEvent_t e : g->prof_events) (void)hipEventDestroy(e);
+    g->prof_events.clear();
+    g->prof_phase.clear();
+  }
+  for (int k = 0; k < MD_PHASE_COUNT; ++k) ms[k] /= steps;
+  for (md_system *s : g->reps) s->steps_done += steps;
+  for (md_system *s : g->reps) check_control(s);
+}
+
+struct md_batch {
+  md_group *group = nullptr;
+};
 
 // ---- the C ABI -------------------------------------------------------------
 
@@ -762,7 +894,7 @@ md_err md_constrain_positions(md_system *s, double tolerance) {
   MD_API_BEGIN
   if (!s || !(tolerance > 0)) throw md_error(MD_ERR_INVALID_ARGUMENT, "bad arguments to md_constrain_positions");
   MD_HIP(hipSetDevice(s->device));
-  md_constrain(s, 0, tolerance);
+  md_constrain(s->self, 0, tolerance);
   MD_HIP(hipStreamSynchronize(s->stream));
   s->need_rebuild = true;
   MD_API_END
@@ -772,7 +904,7 @@ md_err md_constrain_velocities(md_system *s, double tolerance) {
   MD_API_BEGIN
   if (!s || !(tolerance > 0)) throw md_error(MD_ERR_INVALID_ARGUMENT, "bad arguments to md_constrain_velocities");
   MD_HIP(hipSetDevice(s->device));
-  md_constrain(s, 1, tolerance);
+  md_constrain(s->self, 1, tolerance);
   MD_HIP(hipStreamSynchronize(s->stream));
   s->momentum_dirty = true;
   MD_API_END
@@ -819,38 +951,7 @@ const char *md_phase_name(int32_t phase) {
 md_err md_profile_steps(md_system *s, int32_t steps, double *ms) {
   MD_API_BEGIN
   if (!s || !ms || steps <= 0) throw md_error(MD_ERR_INVALID_ARGUMENT, "bad arguments to md_profile_steps");
-  MD_HIP(hipSetDevice(s->device));
-  for (int k = 0; k < MD_PHASE_COUNT; ++k) ms[k] = 0.0;
-  MD_HIP(hipMemcpyAsync(&s->ctl->step_next, &s->steps_done, sizeof(int64_t), hipMemcpyHostToDevice, s->stream));
-  if (s->momentum_dirty) {
-    md_momentum(s);
-    s->momentum_dirty = false;
-  }
-  md_zero_forces(s, false);
-  for (int r = 0; r < steps; ++r) {
-    s->profiling = true;
-    try {
-      launch_step(s, s->need_rebuild);
-    } catch (...) {
-      s->profiling = false;
-      throw;
-    }
-    s->profiling = false;
-    s->need_rebuild = false;
-    MD_HIP(hipStreamSynchronize(s->stream));
-    for (size_t k = 0; k + 1 < s->prof_events.size(); ++k) {
-      float t = 0.0f;
-      MD_HIP(hipEventElapsedTime(&t, s->prof_events[k], s->prof_events[k + 1]));
-      int ph = s->prof_phase[k];
-      if (ph >= 0 && ph < MD_PHASE_COUNT) ms[ph] += t;
-    }
-    for (hipEvent_t e : s->prof_events) (void)hipEventDestroy(e);
-    s->prof_events.clear();
-    s->prof_phase.clear();
-  }
-  for (int k = 0; k < MD_PHASE_COUNT; ++k) ms[k] /= steps;
-  s->steps_done += steps;
-  check_control(s);
+  profile_group(s->self, steps, ms);
   MD_API_END
 }
 
@@ -921,6 +1022,158 @@ md_err md_inject_fault(md_system *s, int32_t after) {
   MD_API_BEGIN
   if (!s) throw md_error(MD_ERR_INVALID_ARGUMENT, "null handle");
   s->fault_after = after;
+  MD_API_END
+}
+
+md_err md_batch_create(md_system *const *systems, int32_t count, md_batch **out) {
+  MD_API_BEGIN
+  if (!out || !systems || count < 1) throw md_error(MD_ERR_INVALID_ARGUMENT, "bad arguments to md_batch_create");
+  *out = nullptr;
+  std::vector<md_system *> reps(systems, systems + count);
+  for (int r = 0; r < count; ++r) {
+    if (!reps[r]) throw md_error(MD_ERR_INVALID_ARGUMENT, "null system in a batch");
+    if (reps[r]->device != reps[0]->device)
+      throw md_error(MD_ERR_INVALID_ARGUMENT, "the systems of a batch must share one device");
+    for (int q = 0; q < r; ++q)
+      if (reps[q] == reps[r]) throw md_error(MD_ERR_INVALID_ARGUMENT, "a system appears twice in a batch");
+  }
+  md_batch *b = new md_batch();
+  try {
+    b->group = group_create(reps, nullptr);
+    b->group->owner = &b->group;
+  } catch (...) {
+    delete b;
+    throw;
+  }
+  *out = b;
+  MD_API_END
+}
+
+void md_batch_destroy(md_batch *b) {
+  if (!b) return;
+  if (b->group) group_release(b->group);
+  delete b;
+}
+
+md_err md_batch_run(md_batch *b, int64_t steps) {
+  MD_API_BEGIN
+  if (!b || steps < 0) throw md_error(MD_ERR_INVALID_ARGUMENT, "bad arguments to md_batch_run");
+  if (!b->group) throw md_error(MD_ERR_INVALID_ARGUMENT, "a system of this batch has been destroyed");
+  run_group(b->group, steps);
+  MD_API_END
+}
+
+md_err md_batch_profile_steps(md_batch *b, int32_t steps, double *ms) {
+  MD_API_BEGIN
+  if (!b || !ms || steps <= 0) throw md_error(MD_ERR_INVALID_ARGUMENT, "bad arguments to md_batch_profile_steps");
+  if (!b->group) throw md_error(MD_ERR_INVALID_ARGUMENT, "a system of this batch has been destroyed");
+  profile_group(b->group, steps, ms);
+  MD_API_END
+}
+
+// ---- learned SHAKE warm start ----------------------------------------------
+
+md_err md_warmstart_create(int32_t device, uint32_t seed, int64_t sample_capacity, md_warmstart **out) {
+  MD_API_BEGIN
+  if (!out || sample_capacity < 0) throw md_error(MD_ERR_INVALID_ARGUMENT, "bad arguments to md_warmstart_create");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw md_error(MD_ERR_NO_DEVICE, "no HIP device");
+  if (device < 0 || device >= ndev) throw md_error(MD_ERR_INVALID_ARGUMENT, "no such HIP device");
+  *out = md_ws_create(device, seed, sample_capacity);
+  MD_API_END
+}
+
+void md_warmstart_destroy(md_warmstart *ws) { md_ws_destroy(ws); }
+
+int32_t md_warmstart_param_count(void) { return MD_WS_NPARAMS; }
+
+static void refresh_all(md_system *s) {
+  for (md_group *g : s->groups) group_refresh(g);
+}
+
+md_err md_warmstart_record(md_warmstart *ws, md_system *s, int64_t steps) {
+  MD_API_BEGIN
+  if (!ws || !s || steps < 0) throw md_error(MD_ERR_INVALID_ARGUMENT, "bad arguments to md_warmstart_record");
+  if (md_ws_device(ws) != s->device) throw md_error(MD_ERR_INVALID_ARGUMENT, "the network and the system are on different devices");
+  if (s->n_clusters_c == 0 || steps == 0) return MD_OK;
+  float *xs, *ts;
+  int64_t room;
+  md_ws_reserve(ws, steps * s->n_clusters_c, &xs, &ts, &room);
+  int64_t fit = std::min<int64_t>(steps, room / s->n_clusters_c);
+  if (fit <= 0) return MD_OK;
+  s->rec_x = xs;
+  s->rec_t = ts;
+  s->rec_first_step = s->steps_done;
+  s->rec_cap = fit * s->n_clusters_c;
+  try {
+    refresh_all(s);
+    run_steps(s, fit);
+  } catch (...) {
+    s->rec_x = s->rec_t = nullptr;
+    s->rec_cap = 0;
+    refresh_all(s);
+    throw;
+  }
+  md_ws_commit(ws, fit * s->n_clusters_c);
+  s->rec_x = s->rec_t = nullptr;
+  s->rec_cap = 0;
+  refresh_all(s);
+  MD_API_END
+}
+
+md_err md_warmstart_sample_count(md_warmstart *ws, int64_t *count) {
+  MD_API_BEGIN
+  if (!ws || !count) throw md_error(MD_ERR_INVALID_ARGUMENT, "null argument");
+  *count = md_ws_count(ws);
+  MD_API_END
+}
+
+md_err md_warmstart_train(md_warmstart *ws, int32_t iterations, int32_t batch, double learning_rate, double *loss) {
+  MD_API_BEGIN
+  if (!ws || iterations < 0 || batch <= 0 || !(learning_rate > 0))
+    throw md_error(MD_ERR_INVALID_ARGUMENT, "bad arguments to md_warmstart_train");
+  double l = iterations > 0 ? md_ws_train(ws, iterations, batch, learning_rate) : 0.0;
+  if (loss) *loss = l;
+  MD_API_END
+}
+
+md_err md_warmstart_attach(md_system *s, md_warmstart *ws) {
+  MD_API_BEGIN
+  if (!s) throw md_error(MD_ERR_INVALID_ARGUMENT, "null system");
+  if (ws && md_ws_device(ws) != s->device)
+    throw md_error(MD_ERR_INVALID_ARGUMENT, "the network and the system are on different devices");
+  MD_HIP(hipSetDevice(s->device));
+  s->ws_net = ws ? md_ws_device_net(ws) : nullptr;
+  refresh_all(s);
+  MD_API_END
+}
+
+md_err md_warmstart_params(md_warmstart *ws, float *get, const float *set) {
+  MD_API_BEGIN
+  if (!ws || (!get && !set)) throw md_error(MD_ERR_INVALID_ARGUMENT, "bad arguments to md_warmstart_params");
+  if (set) md_ws_set_params(ws, set);
+  if (get) md_ws_get_params(ws, get);
+  MD_API_END
+}
+
+md_err md_warmstart_evaluate(md_warmstart *ws, const float *x, const float *t, int32_t n, float *y, float *grad,
+                             double *loss) {
+  MD_API_BEGIN
+  if (!ws || n < 0 || (n > 0 && (!x || !t))) throw md_error(MD_ERR_INVALID_ARGUMENT, "bad arguments to md_warmstart_evaluate");
+  double l = md_ws_evaluate(ws, x, t, n, y, grad);
+  if (loss) *loss = l;
+  MD_API_END
+}
+
+md_err md_shake_counts(md_system *s, int64_t *passes, int64_t *clusters) {
+  MD_API_BEGIN
+  if (!s) throw md_error(MD_ERR_INVALID_ARGUMENT, "null system");
+  md_control c;
+  MD_HIP(hipSetDevice(s->device));
+  MD_HIP(hipMemcpy(&c, s->ctl, sizeof c, hipMemcpyDeviceToHost));
+  if (passes) *passes = c.shake_passes;
+  if (clusters) *clusters = c.shake_clusters;
   MD_API_END
 }
 

@@ -49,8 +49,10 @@ __device__ __forceinline__ void md_diff(const md_term_args &A, int from, int to,
 }
 
 template <bool ENERGY>
-__global__ void __launch_bounds__(256) k_terms(md_term_args A) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_terms(const md_term_args *__restrict__ T, md_bmap map) {
+  int blk;
+  const md_term_args &A = T[md_locate(map, blockIdx.x, blk)];
+  int t = blk * blockDim.x + threadIdx.x;
   int total = A.n_bonds + A.n_angles + A.n_torsions + A.n_exceptions + A.n_restraints;
   double e_bond = 0, e_angle = 0, e_torsion = 0, e_exc = 0, e_res = 0;
   if (t < total) {
@@ -178,21 +180,15 @@ __global__ void __launch_bounds__(256) k_terms(md_term_args A) {
   }
 }
 
-__global__ void k_zero_forces(int64_t *force, int count, int64_t *energy, int zero_energy) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) force[i] = 0;
-  if (zero_energy && i < MD_E_SLOTS) energy[i] = 0;
+__global__ void k_zero_forces(const md_term_args *__restrict__ T, md_bmap map, int zero_energy) {
+  int blk;
+  const md_term_args &A = T[md_locate(map, blockIdx.x, blk)];
+  int i = blk * blockDim.x + threadIdx.x;
+  if (i < 3 * A.np) A.force[i] = 0;
+  if (zero_energy && i < MD_E_SLOTS) A.energy[i] = 0;
 }
 
-void md_zero_forces(md_system *s, bool energies) {
-  int count = 3 * s->np;
-  k_zero_forces<<<(count + 255) / 256, 256, 0, s->stream>>>(s->force, count, s->energy, energies ? 1 : 0);
-  MD_HIP(hipGetLastError());
-}
-
-void md_bonded(md_system *s, bool energy) {
-  int total = s->n_bonds + s->n_angles + s->n_torsions + s->n_exceptions + s->n_restraints;
-  if (total == 0) return;
+static md_term_args term_args(md_system *s) {
   md_term_args A;
   A.atoms = s->term_atoms;
   A.param = s->term_param;
@@ -208,10 +204,56 @@ void md_bonded(md_system *s, bool energy) {
   A.n_restraints = s->n_restraints;
   A.alpha = s->alpha;
   A.box = s->dbox;
-  int g = (total + 255) / 256;
+  return A;
+}
+
+struct md_bonded_group {
+  md_term_args *table = nullptr;
+  md_launch zero, terms;
+};
+
+static md_bonded_group &bg(md_group *g) { return *reinterpret_cast<md_bonded_group *>(g->bonded); }
+
+void md_bonded_group_setup(md_group *g) {
+  auto *G = new md_bonded_group();
+  g->bonded = G;
+  std::vector<int> zero, terms;
+  for (md_system *s : g->reps) {
+    zero.push_back((3 * s->np + 255) / 256);
+    int total = s->n_bonds + s->n_angles + s->n_torsions + s->n_exceptions + s->n_restraints;
+    terms.push_back((total + 255) / 256);
+  }
+  G->zero = md_make_launch(g, zero);
+  G->terms = md_make_launch(g, terms);
+  G->table = md_group_alloc<md_term_args>(g, g->reps.size());
+  md_bonded_group_refresh(g);
+}
+
+void md_bonded_group_refresh(md_group *g) {
+  std::vector<md_term_args> host;
+  for (md_system *s : g->reps) host.push_back(term_args(s));
+  MD_HIP(hipMemcpyAsync(bg(g).table, host.data(), sizeof(md_term_args) * host.size(), hipMemcpyHostToDevice,
+                        g->stream));
+  MD_HIP(hipStreamSynchronize(g->stream));
+}
+
+void md_bonded_group_free(md_group *g) {
+  delete &bg(g);
+  g->bonded = nullptr;
+}
+
+void md_zero_forces(md_group *g, bool energies) {
+  auto &G = bg(g);
+  k_zero_forces<<<G.zero.total, 256, 0, g->stream>>>(G.table, G.zero.map, energies ? 1 : 0);
+  MD_HIP(hipGetLastError());
+}
+
+void md_bonded(md_group *g, bool energy) {
+  auto &G = bg(g);
+  if (G.terms.total == 0) return;
   if (energy)
-    k_terms<true><<<g, 256, 0, s->stream>>>(A);
+    k_terms<true><<<G.terms.total, 256, 0, g->stream>>>(G.table, G.terms.map);
   else
-    k_terms<false><<<g, 256, 0, s->stream>>>(A);
+    k_terms<false><<<G.terms.total, 256, 0, g->stream>>>(G.table, G.terms.map);
   MD_HIP(hipGetLastError());
 }

@@ -43,8 +43,10 @@ struct md_control {
   uint32_t cluster_reach_bits[2];  // largest cluster radius, super-cluster reach
   int64_t momentum[2][3];          // sum m v in Q32.32, by step parity
   int64_t pair_census;             // in-cutoff pairs, when counted
-  int32_t segment_cursor;          // next list segment for the pair kernel's waves
-  int32_t pad_;
+  uint32_t force_pending;          // the host asks for a rebuild at the next step
+  uint32_t pad_;
+  int64_t shake_passes;            // SHAKE update passes over all cluster-steps
+  int64_t shake_clusters;          // cluster-steps SHAKE has run
 };
 
 struct md_nb_params {
@@ -52,6 +54,92 @@ struct md_nb_params {
   int ncells, gdim[3];    // sorting cells
   float cutoff2, list_cutoff2, half_skin2;
 };
+
+// ---- launch groups ------------------------------------------------------------
+//
+// Every kernel runs over a group of R systems ("replicas") in one launch.  Its
+// arguments sit in a device table, one struct per replica, and a block map
+// gives each replica a contiguous run of the launch's blocks: replica r owns
+// blocks [start[r], start[r+1]).  A block finds its replica by a binary search
+// of the map (R = 1: no search).  A system on its own is a group of one; a
+// batch of replicas is one group, so each phase of a step is one launch
+// however many replicas it carries.  The tables live in device memory, so a
+// captured graph stays valid when a replica's box (and its table entry)
+// changes.
+struct md_bmap {
+  const int *start;  // [R + 1]
+  int R;
+};
+
+__device__ __forceinline__ int md_locate(const md_bmap &m, int blk, int &local) {
+  int lo = 0, hi = m.R;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (m.start[mid] <= blk)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  local = blk - m.start[lo];
+  return lo;
+}
+
+struct md_system;
+struct md_ws_net;
+struct md_warmstart;
+
+// Host side of a block map: per-replica block counts -> device prefix.
+struct md_launch {
+  int total = 0;
+  md_bmap map{nullptr, 0};
+};
+
+struct md_group {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::vector<md_system *> reps;
+  md_group **owner = nullptr;  // cleared when the group is released
+  int max_cluster_atoms = 0;
+  // per-subsystem launch tables (each file's private struct)
+  void *nb = nullptr, *bonded = nullptr, *pme = nullptr, *integ = nullptr;
+  // captured stride
+  hipGraphExec_t stride_exec = nullptr;
+  hipGraph_t stride_graph = nullptr;
+  int stride_steps = 0;
+  int max_batch = 0;  // steps per captured graph (0: the default)
+  int ws_mode = 0;          // cluster integrator: 0 plain, 1 recording, 2 learned
+  bool graph_stale = false; // the captured stride no longer matches the kernels
+  // profiling: events recorded by md_mark while `profiling` is set
+  bool profiling = false;
+  std::vector<hipEvent_t> prof_events;
+  std::vector<int> prof_phase;
+  std::vector<void *> allocations;
+};
+
+// device memory owned by the group, zero-filled
+template <typename T>
+T *md_group_alloc(md_group *g, size_t count) {
+  void *p = nullptr;
+  size_t bytes = count * sizeof(T);
+  if (bytes == 0) bytes = 256;
+  MD_HIP(hipMalloc(&p, bytes));
+  MD_HIP(hipMemsetAsync(p, 0, bytes, g->stream));
+  g->allocations.push_back(p);
+  return static_cast<T *>(p);
+}
+
+// a block map from per-replica block counts (uploaded on the group's stream)
+md_launch md_make_launch(md_group *g, const std::vector<int> &blocks);
+// a device copy of a host vector, owned by the group; waits for the copy, so
+// the caller may free `v` as soon as this returns
+template <typename T>
+T *md_group_upload(md_group *g, const std::vector<T> &v) {
+  T *d = md_group_alloc<T>(g, v.size());
+  if (!v.empty()) MD_HIP(hipMemcpyAsync(d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, g->stream));
+  MD_HIP(hipStreamSynchronize(g->stream));
+  return d;
+}
 
 struct md_system {
   int device = 0;
@@ -123,17 +211,17 @@ struct md_system {
   int *free_atoms = nullptr;
   int max_cluster_atoms = 0;
 
+  // ---- learned SHAKE warm start ------------------------------------------
+  const md_ws_net *ws_net = nullptr;  // the network the integrator runs, or none
+  float *rec_x = nullptr, *rec_t = nullptr;  // sample ring of a recording run
+  int64_t rec_first_step = 0, rec_cap = 0;
+
   // ---- run state ---------------------------------------------------------
-  hipGraphExec_t stride_exec = nullptr;
-  hipGraph_t stride_graph = nullptr;
-  int stride_steps = 0;
+  md_group *self = nullptr;          // this system as a group of one
+  std::vector<md_group *> groups;    // every group it is in (its own first)
   int64_t steps_done = 0;
   bool need_rebuild = true;
   bool momentum_dirty = true;
-  // profiling: events recorded by md_mark while `profiling` is set
-  bool profiling = false;
-  std::vector<hipEvent_t> prof_events;
-  std::vector<int> prof_phase;
 
   std::vector<void *> allocations;
 };
@@ -152,22 +240,53 @@ T *md_alloc(md_system *s, size_t count) {
 
 // Profiling phases (mdcore.h MD_PHASE_*): a launcher calls md_mark when it
 // starts a phase; outside md_profile_steps it does nothing.
-void md_mark(md_system *s, int phase);
+void md_mark(md_group *g, int phase);
 
-// ---- launchers (each in its own .hip file) ----------------------------------
-void md_nb_setup(md_system *s);                 // nonbonded.hip
+// ---- per-system setup (each in its own .hip file) ---------------------------
+void md_nb_setup(md_system *s);                 // nonbonded.hip: buffers of one system
 std::vector<int> md_cell_walk(int W, int H, int D);  // nonbonded.hip: cell sort order
 void md_nb_free(md_system *s);
-void md_nb_build(md_system *s, bool forced);    // rebuild kernels, device-gated
-void md_nb_pairs(md_system *s, bool energy, bool census);
 int64_t md_nb_entries(md_system *s);
 int64_t md_nb_capacity(md_system *s);
-void md_bonded(md_system *s, bool energy);      // bonded.hip
-void md_zero_forces(md_system *s, bool energies);
 void md_pme_setup(md_system *s);                // pme.hip
 void md_pme_free(md_system *s);
-void md_pme(md_system *s, bool energy);
-void md_step_prologue(md_system *s, bool advance, bool force_rebuild);  // integrate.hip
-void md_integrate(md_system *s);
-void md_momentum(md_system *s);
-void md_constrain(md_system *s, int what, double tolerance);  // 0 positions, 1 velocities
+
+// ---- group tables: built once per group, refreshed when a box changes -------
+void md_nb_group_setup(md_group *g);
+void md_nb_group_refresh(md_group *g);
+void md_nb_group_free(md_group *g);
+void md_bonded_group_setup(md_group *g);
+void md_bonded_group_refresh(md_group *g);
+void md_bonded_group_free(md_group *g);
+void md_pme_group_setup(md_group *g);
+void md_pme_group_refresh(md_group *g);
+void md_pme_group_free(md_group *g);
+void md_int_group_setup(md_group *g);
+void md_int_group_refresh(md_group *g);
+void md_int_group_free(md_group *g);
+
+// ---- group launchers -----------------------------------------------------------
+void md_nb_build(md_group *g);                  // rebuild kernels, gated per replica
+void md_nb_pairs(md_group *g, bool energy);
+void md_bonded(md_group *g, bool energy);       // bonded.hip
+void md_zero_forces(md_group *g, bool energies);
+void md_pme(md_group *g, bool energy);          // pme.hip
+// integrate.hip: advance the step counters and latch each replica's rebuild
+// request (its own displacement test, or `force_pending` set by the host)
+void md_step_prologue(md_group *g, bool advance);
+void md_integrate(md_group *g);
+void md_momentum(md_group *g);
+void md_constrain(md_group *g, int what, double tolerance);  // 0 positions, 1 velocities
+
+// ---- learned SHAKE warm start (warmstart.hip) ---------------------------------
+md_warmstart *md_ws_create(int device, uint32_t seed, int64_t capacity);
+void md_ws_destroy(md_warmstart *w);
+const md_ws_net *md_ws_device_net(md_warmstart *w);
+int md_ws_device(md_warmstart *w);
+void md_ws_reserve(md_warmstart *w, int64_t wanted, float **xs, float **ts, int64_t *room);
+void md_ws_commit(md_warmstart *w, int64_t added);
+int64_t md_ws_count(md_warmstart *w);
+void md_ws_get_params(md_warmstart *w, float *out);
+void md_ws_set_params(md_warmstart *w, const float *in);
+double md_ws_evaluate(md_warmstart *w, const float *x, const float *t, int n, float *y, float *grad);
+double md_ws_train(md_warmstart *w, int iterations, int batch, double lr);

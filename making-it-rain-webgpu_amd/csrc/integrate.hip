@@ -22,6 +22,7 @@
 #include <cmath>
 
 #include "engine.hpp"
+#include "warmstart.hpp"
 
 struct md_int_args {
   int64_t *pos;
@@ -50,6 +51,14 @@ struct md_int_args {
   const int4 *ccl_cons;
   const int *free_atoms;
   int n_waters, n_ccl, n_free;
+  int n;  // atoms
+  // learned SHAKE warm start: the network (null: plain SHAKE), and the
+  // sample ring a recording run fills with (features, multipliers)
+  const md_ws_net *ws;
+  float *rec_x;    // [rec_cap][MD_WS_F]
+  float *rec_t;    // [rec_cap][MD_WS_MAXC]
+  int64_t rec_first_step;
+  int64_t rec_cap;
 };
 
 // The three thermostat deviates of one particle: counters (2p, step) and
@@ -217,7 +226,8 @@ __device__ void md_rattle(const md_int_args &A, md_item<MAXA> &it, int nc, const
 // SHAKE (src/reference/constraints.ts:130-186): Jacobi along the reference
 // bond vectors, converged when |r'|^2 lies in [(1-t)^2, (1+t)^2] d^2
 template <int MAXA>
-__device__ void md_shake(const md_int_args &A, md_item<MAXA> &it, int nc, const int4 *cons) {
+__device__ void md_shake(const md_int_args &A, md_item<MAXA> &it, int nc, const int4 *cons,
+                         const float *lam0 = nullptr, double *lam_total = nullptr, int *passes = nullptr) {
   double rv[MD_MAX_CLUSTER_CONSTRAINTS][3], mu[MD_MAX_CLUSTER_CONSTRAINTS], t2[MD_MAX_CLUSTER_CONSTRAINTS];
   double t[MAXA][3], moved[MAXA][3];
   for (int a = 0; a < it.n; ++a)
@@ -235,17 +245,13 @@ __device__ void md_shake(const md_int_args &A, md_item<MAXA> &it, int nc, const 
   }
   double lower = 1.0 - 2.0 * A.tol + A.tol * A.tol, upper = 1.0 + 2.0 * A.tol + A.tol * A.tol;
   double delta[MD_MAX_CLUSTER_CONSTRAINTS];
-  for (int iter = 0; iter < MD_MAX_CONSTRAINT_ITERATIONS; ++iter) {
-    int conv = 0;
-    for (int c = 0; c < nc; ++c) {
-      int li = cons[c].x, lj = cons[c].y;
-      double x = t[li][0] - t[lj][0], y = t[li][1] - t[lj][1], z = t[li][2] - t[lj][2];
-      double l2 = x * x + y * y + z * z;
-      double proj = x * rv[c][0] + y * rv[c][1] + z * rv[c][2];
-      delta[c] = proj == 0.0 ? 0.0 : mu[c] * (t2[c] - l2) / proj;
-      conv += (l2 >= lower * t2[c] && l2 <= upper * t2[c]);
-    }
-    if (conv == nc) break;
+  int applied = 0;
+  for (int c = 0; c < nc; ++c) delta[c] = 0.0;
+  if (lam0) {
+    // the predicted multipliers (in units of each constraint's reduced-mass
+    // factor mu), applied as one update; the passes below correct whatever
+    // the prediction missed
+    for (int c = 0; c < nc && c < MD_WS_MAXC; ++c) delta[c] = mu[c] * (double)lam0[c];
     for (int c = 0; c < nc; ++c) {
       int li = cons[c].x, lj = cons[c].y;
       for (int d = 0; d < 3; ++d) {
@@ -257,8 +263,83 @@ __device__ void md_shake(const md_int_args &A, md_item<MAXA> &it, int nc, const 
       }
     }
   }
+  double total[MD_MAX_CLUSTER_CONSTRAINTS];
+  for (int c = 0; c < nc; ++c) total[c] = delta[c];
+  for (int iter = 0; iter < MD_MAX_CONSTRAINT_ITERATIONS; ++iter) {
+    int conv = 0;
+    for (int c = 0; c < nc; ++c) {
+      int li = cons[c].x, lj = cons[c].y;
+      double x = t[li][0] - t[lj][0], y = t[li][1] - t[lj][1], z = t[li][2] - t[lj][2];
+      double l2 = x * x + y * y + z * z;
+      double proj = x * rv[c][0] + y * rv[c][1] + z * rv[c][2];
+      delta[c] = proj == 0.0 ? 0.0 : mu[c] * (t2[c] - l2) / proj;
+      conv += (l2 >= lower * t2[c] && l2 <= upper * t2[c]);
+    }
+    if (conv == nc) break;
+    ++applied;
+    for (int c = 0; c < nc; ++c) {
+      int li = cons[c].x, lj = cons[c].y;
+      total[c] += delta[c];
+      for (int d = 0; d < 3; ++d) {
+        double dr = rv[c][d] * delta[c];
+        t[li][d] += dr * it.im[li];
+        t[lj][d] -= dr * it.im[lj];
+        moved[li][d] += dr * it.im[li];
+        moved[lj][d] -= dr * it.im[lj];
+      }
+    }
+  }
   for (int a = 0; a < it.n; ++a)
     for (int d = 0; d < 3; ++d) it.tr[a][d] += (int64_t)rint(moved[a][d] * 4294967296.0);
+  if (passes) *passes = applied;
+  if (lam_total)
+    for (int c = 0; c < nc && c < MD_WS_MAXC; ++c) lam_total[c] = mu[c] == 0.0 ? 0.0 : total[c] / mu[c];
+}
+
+// The network's inputs for one cluster of up to MD_WS_MAXC constraints, from
+// the positions at the start of the step (it.x) and the unconstrained trial
+// positions (it.tr): for each constraint c, f[4c..4c+3] = (100 (d^2 - |r'|^2)
+// / d^2, 100 (r'.r / d^2 - 1), m_i^-1 / (m_i^-1 + m_j^-1), 1) with r the
+// reference and r' the trial bond vector; f[16..21] couple the six pairs of
+// constraints that share an atom: +-m_shared^-1 / sqrt(the two inverse-mass
+// sums) times the cosine between their reference vectors.  All invariant
+// under rotation and translation.
+template <int MAXA>
+__device__ void md_ws_features(const md_item<MAXA> &it, int nc, const int4 *cons, float *f) {
+  for (int k = 0; k < MD_WS_F; ++k) f[k] = 0.0f;
+  double rv[MD_WS_MAXC][3], d2[MD_WS_MAXC], isum[MD_WS_MAXC];
+  int n = min(nc, MD_WS_MAXC);
+  for (int c = 0; c < n; ++c) {
+    int li = cons[c].x, lj = cons[c].y;
+    double rp[3], l2 = 0.0, pr = 0.0;
+    for (int d = 0; d < 3; ++d) {
+      rv[c][d] = md_fx(it.x[li][d] - it.x[lj][d]);
+      rp[d] = md_fx(it.tr[li][d] - it.tr[lj][d]);
+      l2 += rp[d] * rp[d];
+      pr += rp[d] * rv[c][d];
+    }
+    double dist = __hiloint2double(cons[c].w, cons[c].z);
+    d2[c] = dist * dist;
+    isum[c] = (double)it.im[li] + (double)it.im[lj];
+    f[4 * c + 0] = (float)(MD_WS_SCALE * (d2[c] - l2) / d2[c]);
+    f[4 * c + 1] = (float)(MD_WS_SCALE * (pr / d2[c] - 1.0));
+    f[4 * c + 2] = isum[c] > 0.0 ? (float)(it.im[li] / isum[c]) : 0.0f;
+    f[4 * c + 3] = 1.0f;
+  }
+  int pair = 0;
+  for (int c = 0; c < MD_WS_MAXC; ++c)
+    for (int e = c + 1; e < MD_WS_MAXC; ++e, ++pair) {
+      if (e >= n) continue;
+      int ci = cons[c].x, cj = cons[c].y, ei = cons[e].x, ej = cons[e].y;
+      double shared = 0.0;
+      if (ci == ei) shared += it.im[ci];
+      if (cj == ej) shared += it.im[cj];
+      if (ci == ej) shared -= it.im[ci];
+      if (cj == ei) shared -= it.im[cj];
+      if (shared == 0.0 || isum[c] == 0.0 || isum[e] == 0.0) continue;
+      double dot = rv[c][0] * rv[e][0] + rv[c][1] * rv[e][1] + rv[c][2] * rv[e][2];
+      f[16 + pair] = (float)(shared / sqrt(isum[c] * isum[e]) * dot / sqrt(d2[c] * d2[e]));
+    }
 }
 
 // SETTLE positions (src/reference/settle.ts:31-196): rigid rotation about the
@@ -399,20 +480,28 @@ __device__ __forceinline__ void md_reduce_out(const md_int_args &A, uint32_t ste
   }
 }
 
-// kind 0: free atoms, 1: SETTLE waters, 2: SHAKE clusters (MAXA atoms)
-template <int KIND, int MAXA>
-__global__ void __launch_bounds__(256) k_integrate(md_int_args A) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+// kind 0: free atoms, 1: SETTLE waters, 2: SHAKE clusters (MAXA atoms).
+// MODE (clusters only): 0 plain SHAKE; 1 plain SHAKE, recording each
+// cluster's features and converged multipliers as training samples; 2 SHAKE
+// started from the network's predicted multipliers.  Every cluster step adds
+// its SHAKE passes to the replica's counters.
+template <int KIND, int MAXA, int MODE = 0>
+__global__ void __launch_bounds__(256) k_integrate(const md_int_args *__restrict__ T, md_bmap map) {
+  int blk;
+  const md_int_args &A = T[md_locate(map, blockIdx.x, blk)];
+  int t = blk * blockDim.x + threadIdx.x;
   uint32_t step = (uint32_t)A.ctl->step_current;
   float vcom[3];
   md_com_velocity(A, step, vcom);
   double mom[3] = {0.0, 0.0, 0.0};
   float worst = 0.0f;
   int count = KIND == 0 ? A.n_free : (KIND == 1 ? A.n_waters : A.n_ccl);
-  if (t < count) {
-    md_item<MAXA> it;
-    int nc = 0;
-    int4 cons[KIND == 2 ? MD_MAX_CLUSTER_CONSTRAINTS : 1];
+  const bool live = t < count;
+  md_item<MAXA> it;
+  it.n = 0;
+  int nc = 0;
+  int4 cons[KIND == 2 ? MD_MAX_CLUSTER_CONSTRAINTS : 1];
+  if (live) {
     if (KIND == 0) {
       it.n = 1;
       it.atom[0] = A.free_atoms[t];
@@ -440,12 +529,53 @@ __global__ void __launch_bounds__(256) k_integrate(md_int_args A) {
     } else if constexpr (KIND == 2) {
       if (A.integrator == 0) md_rattle(A, it, nc, cons);
       md_drift(A, it, step);
-      md_shake(A, it, nc, cons);
     } else {
       md_drift(A, it, step);
     }
-    md_finish(A, it, mom, worst);
   }
+  if constexpr (KIND == 2) {
+    const bool covered = live && nc > 0 && nc <= MD_WS_MAXC;
+    float lam0[MD_WS_MAXC] = {0.0f, 0.0f, 0.0f, 0.0f};
+    bool warm = false;
+    if constexpr (MODE == 2) {
+      // every lane of the workgroup runs the network (its replica's, or none)
+      __shared__ float ws_lds[4 * MD_WS_TILE_LDS];
+      if (A.ws) {
+        float f[MD_WS_F], y[MD_WS_MAXC];
+        if (covered)
+          md_ws_features(it, nc, cons, f);
+        else
+          for (int k = 0; k < MD_WS_F; ++k) f[k] = 0.0f;
+        md_ws_infer(*A.ws, f, y, ws_lds + (threadIdx.x >> 6) * MD_WS_TILE_LDS);
+        if (covered) {
+          for (int c = 0; c < MD_WS_MAXC; ++c) lam0[c] = y[c] * (1.0f / MD_WS_SCALE);
+          warm = true;
+        }
+      }
+    }
+    int passes = 0;
+    if (live) {
+      double total[MD_WS_MAXC];
+      float f[MD_WS_F];
+      if (MODE == 1 && covered) md_ws_features(it, nc, cons, f);
+      md_shake(A, it, nc, cons, warm ? lam0 : nullptr, MODE == 1 ? total : nullptr, &passes);
+      if (MODE == 1 && covered) {
+        int64_t k = (A.ctl->step_current - A.rec_first_step) * (int64_t)A.n_ccl + t;
+        if (k >= 0 && k < A.rec_cap) {
+          for (int q = 0; q < MD_WS_F; ++q) A.rec_x[k * MD_WS_F + q] = f[q];
+          for (int c = 0; c < MD_WS_MAXC; ++c)
+            A.rec_t[k * MD_WS_MAXC + c] = c < nc ? (float)(MD_WS_SCALE * total[c]) : 0.0f;
+        }
+      }
+    }
+    // SHAKE passes and clusters of this step, for the warm start's gain
+    int64_t ps = md_wave_sum_i64((int64_t)passes), cs = md_wave_sum_i64(live ? 1 : 0);
+    if ((threadIdx.x & 63) == 0 && cs) {
+      md_atomic_add_i64(&A.ctl->shake_passes, ps);
+      md_atomic_add_i64(&A.ctl->shake_clusters, cs);
+    }
+  }
+  if (live) md_finish(A, it, mom, worst);
   md_reduce_out(A, step, mom, worst);
 }
 
@@ -454,8 +584,11 @@ __global__ void __launch_bounds__(256) k_integrate(md_int_args A) {
 // reference (constrainPositions(x, x)); WHAT 1 removes every constrained
 // relative velocity (constrainVelocities).
 template <int KIND, int MAXA, int WHAT>
-__global__ void __launch_bounds__(256) k_constrain(md_int_args A) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_constrain(const md_int_args *__restrict__ T, md_bmap map, double tol) {
+  int blk;
+  md_int_args A = T[md_locate(map, blockIdx.x, blk)];
+  A.tol = tol;
+  int t = blk * blockDim.x + threadIdx.x;
   int count = KIND == 1 ? A.n_waters : A.n_ccl;
   if (t >= count) return;
   md_item<MAXA> it;
@@ -508,35 +641,45 @@ __global__ void __launch_bounds__(256) k_constrain(md_int_args A) {
   }
 }
 
-// start of a step: advance the device step counter, latch the rebuild
-// request, clear the momentum slot this step's integrator fills
-__global__ void k_step_prologue(md_control *ctl, int advance, int force_rebuild) {
-  if (threadIdx.x != 0) return;
+// start of a step, one thread per replica: advance the device step counter,
+// latch the rebuild request (the displacement test's, or the host's),
+// clear the momentum slot this step's integrator fills
+__global__ void k_step_prologue(const md_int_args *__restrict__ T, int R, int advance) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  md_control *ctl = T[r].ctl;
   if (advance) {
     ctl->step_current = ctl->step_next;
     ctl->step_next = ctl->step_next + 1;
     int64_t s = ctl->step_current;
     for (int d = 0; d < 3; ++d) ctl->momentum[(s + 1) & 1][d] = 0;
   }
-  ctl->rebuild_active = ctl->rebuild_request[0] | (uint32_t)force_rebuild;
+  ctl->rebuild_active = ctl->rebuild_request[0] | ctl->force_pending;
   ctl->rebuild_request[0] = 0;
+  ctl->force_pending = 0;
 }
 
 // momentum of the current velocities, into the slot the next step reads
-__global__ void k_momentum(int n, int np, const float *vel, const float *mass, md_control *ctl) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_momentum(const md_int_args *__restrict__ T, md_bmap map) {
+  int blk;
+  const md_int_args &A = T[md_locate(map, blockIdx.x, blk)];
+  md_control *ctl = A.ctl;
+  int i = blk * blockDim.x + threadIdx.x;
   uint32_t slot = (uint32_t)(ctl->step_next & 1);
   double m[3] = {0, 0, 0};
-  if (i < n)
-    for (int d = 0; d < 3; ++d) m[d] = (double)mass[i] * vel[d * np + i];
+  if (i < A.n)
+    for (int d = 0; d < 3; ++d) m[d] = (double)A.mass[i] * A.vel[d * A.np + i];
   for (int d = 0; d < 3; ++d) {
     int64_t f = md_wave_sum_i64((int64_t)rint(m[d] * 4294967296.0));
     if ((threadIdx.x & 63) == 0) md_atomic_add_i64(&ctl->momentum[slot][d], f);
   }
 }
 
-__global__ void k_clear_momentum(md_control *ctl) {
-  if (threadIdx.x < 3) ctl->momentum[ctl->step_next & 1][threadIdx.x] = 0;
+__global__ void k_clear_momentum(const md_int_args *__restrict__ T, int R) {
+  int r = blockIdx.x;
+  if (r >= R || threadIdx.x >= 3) return;
+  md_control *ctl = T[r].ctl;
+  ctl->momentum[ctl->step_next & 1][threadIdx.x] = 0;
 }
 
 static md_int_args int_args(md_system *s) {
@@ -574,57 +717,129 @@ static md_int_args int_args(md_system *s) {
   A.n_waters = s->n_waters;
   A.n_ccl = s->n_clusters_c;
   A.n_free = s->n_free;
+  A.n = s->n;
+  A.ws = s->ws_net;
+  A.rec_x = s->rec_x;
+  A.rec_t = s->rec_t;
+  A.rec_first_step = s->rec_first_step;
+  A.rec_cap = s->rec_cap;
   return A;
 }
 
-void md_step_prologue(md_system *s, bool advance, bool force_rebuild) {
-  k_step_prologue<<<1, 64, 0, s->stream>>>(s->ctl, advance ? 1 : 0, force_rebuild ? 1 : 0);
+struct md_int_group {
+  md_int_args *table = nullptr;
+  md_launch free_items, waters, clusters, atoms;
+  int R = 0;
+};
+
+static md_int_group &ig(md_group *g) { return *reinterpret_cast<md_int_group *>(g->integ); }
+
+void md_int_group_setup(md_group *g) {
+  auto *G = new md_int_group();
+  g->integ = G;
+  G->R = (int)g->reps.size();
+  std::vector<int> fr, wa, cl, at;
+  g->max_cluster_atoms = 0;
+  for (md_system *s : g->reps) {
+    fr.push_back((s->n_free + 255) / 256);
+    wa.push_back((s->n_waters + 255) / 256);
+    cl.push_back((s->n_clusters_c + 255) / 256);
+    at.push_back((s->n + 255) / 256);
+    g->max_cluster_atoms = std::max(g->max_cluster_atoms, s->max_cluster_atoms);
+  }
+  G->free_items = md_make_launch(g, fr);
+  G->waters = md_make_launch(g, wa);
+  G->clusters = md_make_launch(g, cl);
+  G->atoms = md_make_launch(g, at);
+  G->table = md_group_alloc<md_int_args>(g, G->R);
+  md_int_group_refresh(g);
+}
+
+void md_int_group_refresh(md_group *g) {
+  std::vector<md_int_args> host;
+  int mode = 0;
+  for (md_system *s : g->reps) {
+    host.push_back(int_args(s));
+    if (s->rec_cap > 0) mode = 1;
+    else if (s->ws_net && mode == 0) mode = 2;
+  }
+  if (mode != g->ws_mode) {
+    g->ws_mode = mode;
+    g->graph_stale = true;  // the cluster kernel changes
+  }
+  MD_HIP(hipMemcpyAsync(ig(g).table, host.data(), sizeof(md_int_args) * host.size(), hipMemcpyHostToDevice,
+                        g->stream));
+  MD_HIP(hipStreamSynchronize(g->stream));
+}
+
+void md_int_group_free(md_group *g) {
+  delete &ig(g);
+  g->integ = nullptr;
+}
+
+void md_step_prologue(md_group *g, bool advance) {
+  auto &G = ig(g);
+  k_step_prologue<<<(G.R + 63) / 64, 64, 0, g->stream>>>(G.table, G.R, advance ? 1 : 0);
   MD_HIP(hipGetLastError());
 }
 
-void md_integrate(md_system *s) {
-  md_int_args A = int_args(s);
-  hipStream_t st = s->stream;
-  if (A.n_free) k_integrate<0, 1><<<(A.n_free + 255) / 256, 256, 0, st>>>(A);
-  if (A.n_waters) k_integrate<1, 3><<<(A.n_waters + 255) / 256, 256, 0, st>>>(A);
-  if (A.n_ccl) {
-    if (s->max_cluster_atoms <= 4)
-      k_integrate<2, 4><<<(A.n_ccl + 255) / 256, 256, 0, st>>>(A);
-    else
-      k_integrate<2, 8><<<(A.n_ccl + 255) / 256, 256, 0, st>>>(A);
-  }
-  MD_HIP(hipGetLastError());
-}
-
-void md_constrain(md_system *s, int what, double tolerance) {
-  md_int_args A = int_args(s);
-  A.tol = tolerance;
-  hipStream_t st = s->stream;
-  if (A.n_waters) {
-    if (what == 0)
-      k_constrain<1, 3, 0><<<(A.n_waters + 255) / 256, 256, 0, st>>>(A);
-    else
-      k_constrain<1, 3, 1><<<(A.n_waters + 255) / 256, 256, 0, st>>>(A);
-  }
-  if (A.n_ccl) {
-    int g = (A.n_ccl + 255) / 256;
-    if (s->max_cluster_atoms <= 4) {
-      if (what == 0)
-        k_constrain<2, 4, 0><<<g, 256, 0, st>>>(A);
+void md_integrate(md_group *g) {
+  auto &G = ig(g);
+  hipStream_t st = g->stream;
+  const md_int_args *T = G.table;
+  if (G.free_items.total) k_integrate<0, 1><<<G.free_items.total, 256, 0, st>>>(T, G.free_items.map);
+  if (G.waters.total) k_integrate<1, 3><<<G.waters.total, 256, 0, st>>>(T, G.waters.map);
+  if (G.clusters.total) {
+    const int mode = g->ws_mode;
+    if (g->max_cluster_atoms <= 4) {
+      if (mode == 2)
+        k_integrate<2, 4, 2><<<G.clusters.total, 256, 0, st>>>(T, G.clusters.map);
+      else if (mode == 1)
+        k_integrate<2, 4, 1><<<G.clusters.total, 256, 0, st>>>(T, G.clusters.map);
       else
-        k_constrain<2, 4, 1><<<g, 256, 0, st>>>(A);
+        k_integrate<2, 4, 0><<<G.clusters.total, 256, 0, st>>>(T, G.clusters.map);
     } else {
-      if (what == 0)
-        k_constrain<2, 8, 0><<<g, 256, 0, st>>>(A);
+      if (mode == 2)
+        k_integrate<2, 8, 2><<<G.clusters.total, 256, 0, st>>>(T, G.clusters.map);
+      else if (mode == 1)
+        k_integrate<2, 8, 1><<<G.clusters.total, 256, 0, st>>>(T, G.clusters.map);
       else
-        k_constrain<2, 8, 1><<<g, 256, 0, st>>>(A);
+        k_integrate<2, 8, 0><<<G.clusters.total, 256, 0, st>>>(T, G.clusters.map);
     }
   }
   MD_HIP(hipGetLastError());
 }
 
-void md_momentum(md_system *s) {
-  k_clear_momentum<<<1, 64, 0, s->stream>>>(s->ctl);
-  k_momentum<<<(s->n + 255) / 256, 256, 0, s->stream>>>(s->n, s->np, s->vel, s->mass_f, s->ctl);
+void md_constrain(md_group *g, int what, double tolerance) {
+  auto &G = ig(g);
+  hipStream_t st = g->stream;
+  const md_int_args *T = G.table;
+  if (G.waters.total) {
+    if (what == 0)
+      k_constrain<1, 3, 0><<<G.waters.total, 256, 0, st>>>(T, G.waters.map, tolerance);
+    else
+      k_constrain<1, 3, 1><<<G.waters.total, 256, 0, st>>>(T, G.waters.map, tolerance);
+  }
+  if (G.clusters.total) {
+    int gsz = G.clusters.total;
+    if (g->max_cluster_atoms <= 4) {
+      if (what == 0)
+        k_constrain<2, 4, 0><<<gsz, 256, 0, st>>>(T, G.clusters.map, tolerance);
+      else
+        k_constrain<2, 4, 1><<<gsz, 256, 0, st>>>(T, G.clusters.map, tolerance);
+    } else {
+      if (what == 0)
+        k_constrain<2, 8, 0><<<gsz, 256, 0, st>>>(T, G.clusters.map, tolerance);
+      else
+        k_constrain<2, 8, 1><<<gsz, 256, 0, st>>>(T, G.clusters.map, tolerance);
+    }
+  }
+  MD_HIP(hipGetLastError());
+}
+
+void md_momentum(md_group *g) {
+  auto &G = ig(g);
+  k_clear_momentum<<<G.R, 64, 0, g->stream>>>(G.table, G.R);
+  k_momentum<<<G.atoms.total, 256, 0, g->stream>>>(G.table, G.atoms.map);
   MD_HIP(hipGetLastError());
 }

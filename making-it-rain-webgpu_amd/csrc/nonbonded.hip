@@ -73,19 +73,72 @@ struct md_build_ctx {
   md_box box;
 };
 
-__device__ __forceinline__ bool md_build_gate(const md_control *ctl, int forced) {
-  return forced || ctl->rebuild_active;
+struct md_list_ptrs {
+  const int64_t *cluster_center;
+  const float4 *cluster_half;
+  const int64_t *super_origin;
+  int *super_count;   // [nsuper+1] entries per super-cluster, then starts
+  int *super_segs;    // [nsuper+1] segments per super-cluster, then starts
+  int *entry_j;
+  int *entry_imask;
+  float4 *entry_shift;
+  int4 *segments;     // (S, begin, end, 0)
+  uint64_t *entry_excl;  // [capacity][8]
+};
+
+struct md_pair_args {
+  const int4 *segments;
+  const int *entry_j;
+  const int *entry_imask;
+  const float4 *entry_shift;
+  const uint64_t *entry_excl;
+  const float4 *slot_xyzq;
+  const float2 *slot_lj;
+  const int *slot_atom;
+  const int *totals;
+  int64_t *force;
+  int64_t *energy;
+  int64_t *census;
+  int np, nclusters;
+  float cutoff2, alpha, beta, exp_scale;
+};
+
+// Everything the nonbonded kernels of one replica read: its table entry.
+struct md_nb_args {
+  md_build_ctx c;
+  md_list_ptrs L;
+  md_pair_args P;
+  const int64_t *pos;
+  const int *cell_rank;
+  int *cell_count, *cell_start, *cell_cursor, *atom_cell, *cell_member;
+  int *slot_atom, *atom_slot;
+  int64_t *folded, *build_pos, *slot_offset, *cluster_center, *super_origin;
+  float2 *slot_lj;
+  float4 *slot_xyzq, *cluster_half;
+  const float4 *atom_param;
+  const int *excl_start, *excl_list;
+  int *totals;
+  md_control *ctl;
+};
+
+// this block's replica entry and its block index inside the replica
+__device__ __forceinline__ const md_nb_args &md_nb_entry(const md_nb_args *T, const md_bmap &map, int &blk) {
+  return T[md_locate(map, blockIdx.x, blk)];
 }
 
 // Fold each atom into the cell and bucket it (cell-list.wgsl:139-250).  The
 // fractional coordinate is computed in f64 from the exact fixed-point
 // position; the folding subtracts whole lattice vectors in Q32.32.
-__global__ void k_nb_assign(md_build_ctx c, const int64_t *__restrict__ pos, const int *__restrict__ cell_rank,
-                            int *cell_count, int *atom_cell, int64_t *folded, int64_t *build_pos,
-                            const md_control *ctl, int forced) {
-  if (!md_build_gate(ctl, forced)) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_nb_assign(const md_nb_args *__restrict__ T, md_bmap map) {
+  int blk;
+  const md_nb_args &A = md_nb_entry(T, map, blk);
+  if (!A.ctl->rebuild_active) return;
+  const md_build_ctx &c = A.c;
+  int i = blk * blockDim.x + threadIdx.x;
   if (i >= c.n) return;
+  const int64_t *pos = A.pos;
+  int *atom_cell = A.atom_cell;
+  int64_t *folded = A.folded, *build_pos = A.build_pos;
   int64_t x = pos[i], y = pos[c.np + i], z = pos[2 * c.np + i];
   const double s = 2.3283064365386963e-10;
   double rx = (double)x * s, ry = (double)y * s, rz = (double)z * s;
@@ -105,9 +158,9 @@ __global__ void k_nb_assign(md_build_ctx c, const int64_t *__restrict__ pos, con
   ga = max(ga, 0);
   gb = max(gb, 0);
   gc = max(gc, 0);
-  int key = cell_rank[(ga * c.gdim1 + gb) * c.gdim2 + gc];
+  int key = A.cell_rank[(ga * c.gdim1 + gb) * c.gdim2 + gc];
   atom_cell[i] = key;
-  atomicAdd(&cell_count[key], 1);
+  atomicAdd(&A.cell_count[key], 1);
   folded[i] = fx;
   folded[c.np + i] = fy;
   folded[2 * c.np + i] = fz;
@@ -143,14 +196,18 @@ __device__ void md_block_scan(const int *in, int *out, int count, int *clear) {
   if (t == 1023) out[count] = partial[1023];
 }
 
-__global__ void __launch_bounds__(1024) k_nb_scan_cells(int ncells, int *cell_count, int *cell_start,
-                                                        int *cell_cursor, md_control *ctl, int forced) {
-  if (!md_build_gate(ctl, forced)) return;
-  md_block_scan(cell_count, cell_start, ncells, nullptr);
+// one workgroup per replica
+__global__ void __launch_bounds__(1024) k_nb_scan_cells(const md_nb_args *__restrict__ T, md_bmap map) {
+  int blk;
+  const md_nb_args &A = md_nb_entry(T, map, blk);
+  md_control *ctl = A.ctl;
+  if (!ctl->rebuild_active) return;
+  int ncells = A.c.ncells;
+  md_block_scan(A.cell_count, A.cell_start, ncells, nullptr);
   __syncthreads();
   for (int k = threadIdx.x; k < ncells; k += 1024) {
-    cell_count[k] = 0;
-    cell_cursor[k] = 0;
+    A.cell_count[k] = 0;
+    A.cell_cursor[k] = 0;
   }
   if (threadIdx.x == 0) {
     ctl->rebuilds += 1;
@@ -160,44 +217,51 @@ __global__ void __launch_bounds__(1024) k_nb_scan_cells(int ncells, int *cell_co
   }
 }
 
-__global__ void k_nb_scatter(int n, const int *__restrict__ atom_cell, const int *__restrict__ cell_start,
-                             int *cell_cursor, int *cell_member, const md_control *ctl, int forced) {
-  if (!md_build_gate(ctl, forced)) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int key = atom_cell[i];
-  int at = cell_start[key] + atomicAdd(&cell_cursor[key], 1);
-  cell_member[at] = i;
+__global__ void k_nb_scatter(const md_nb_args *__restrict__ T, md_bmap map) {
+  int blk;
+  const md_nb_args &A = md_nb_entry(T, map, blk);
+  if (!A.ctl->rebuild_active) return;
+  int i = blk * blockDim.x + threadIdx.x;
+  if (i >= A.c.n) return;
+  int key = A.atom_cell[i];
+  int at = A.cell_start[key] + atomicAdd(&A.cell_cursor[key], 1);
+  A.cell_member[at] = i;
 }
 
 // Deterministic order inside a cell: rank by atom index over the cell's
 // members (the reference's `sort_cells`, cell-list.wgsl:318-331).
-__global__ void k_nb_rank(int n, const int *__restrict__ atom_cell, const int *__restrict__ cell_start,
-                          const int *__restrict__ cell_member, int *slot_atom, int *atom_slot,
-                          const md_control *ctl, int forced) {
-  if (!md_build_gate(ctl, forced)) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int key = atom_cell[i];
-  int b = cell_start[key], e = cell_start[key + 1];
+__global__ void k_nb_rank(const md_nb_args *__restrict__ T, md_bmap map) {
+  int blk;
+  const md_nb_args &A = md_nb_entry(T, map, blk);
+  if (!A.ctl->rebuild_active) return;
+  int i = blk * blockDim.x + threadIdx.x;
+  if (i >= A.c.n) return;
+  int key = A.atom_cell[i];
+  int b = A.cell_start[key], e = A.cell_start[key + 1];
   int rank = 0;
-  for (int k = b; k < e; ++k) rank += cell_member[k] < i;
+  for (int k = b; k < e; ++k) rank += A.cell_member[k] < i;
   int slot = b + rank;
-  slot_atom[slot] = i;
-  atom_slot[i] = slot;
+  A.slot_atom[slot] = i;
+  A.atom_slot[i] = slot;
 }
 
 // Per 8-atom cluster: bounding box of the folded build positions.  Per
 // 64-atom super-cluster: a frame origin, and for each slot the fixed-point
 // offset that takes a current position to f32 coordinates in that frame.
-__global__ void k_nb_clusters(md_build_ctx c, const int *__restrict__ slot_atom,
-                              const int64_t *__restrict__ folded, const int64_t *__restrict__ build_pos,
-                              const float4 *__restrict__ atom_param, int64_t *cluster_center,
-                              float4 *cluster_half, int64_t *super_origin, int64_t *slot_offset,
-                              float2 *slot_lj, md_control *ctl, int forced) {
-  if (!md_build_gate(ctl, forced)) return;
+__global__ void k_nb_clusters(const md_nb_args *__restrict__ T, md_bmap map) {
+  int blk;
+  const md_nb_args &A = md_nb_entry(T, map, blk);
+  md_control *ctl = A.ctl;
+  if (!ctl->rebuild_active) return;
+  const md_build_ctx &c = A.c;
+  const int *slot_atom = A.slot_atom;
+  const int64_t *folded = A.folded, *build_pos = A.build_pos;
+  const float4 *atom_param = A.atom_param;
+  int64_t *cluster_center = A.cluster_center, *super_origin = A.super_origin, *slot_offset = A.slot_offset;
+  float4 *cluster_half = A.cluster_half;
+  float2 *slot_lj = A.slot_lj;
   // one thread per cluster; the 8 threads of a super-cluster cooperate
-  int k = blockIdx.x * blockDim.x + threadIdx.x;  // cluster index
+  int k = blk * blockDim.x + threadIdx.x;  // cluster index
   int lane = threadIdx.x & 63;
   bool live = k < c.nclusters;
   int64_t lo[3] = {INT64_MAX, INT64_MAX, INT64_MAX}, hi[3] = {INT64_MIN, INT64_MIN, INT64_MIN};
@@ -303,22 +367,6 @@ __device__ void md_for_each_image(const md_build_ctx &c, int64_t dx, int64_t dy,
       }
 }
 
-struct md_list_ptrs {
-  const int64_t *cluster_center;
-  const float4 *cluster_half;
-  const int64_t *super_origin;
-  int *super_count;   // [nsuper+1] entries per super-cluster, then starts
-  int *super_segs;    // [nsuper+1] segments per super-cluster, then starts
-  int *entry_j;
-  int *entry_imask;
-  float4 *entry_shift;
-  int4 *segments;     // (S, begin, end, 0)
-  uint64_t *entry_excl;  // [capacity][8]
-};
-
-// Count (pass 0) or write (pass 1) the list entries of one super-cluster: one
-// wavefront per S, lanes over candidate J clusters, compacted with a
-// wavefront prefix count so the order is deterministic.
 // The entries one lane contributes for candidate cluster J: for each passing
 // periodic image, `emit(imask, self, shift)` in a fixed order.
 template <typename F>
@@ -357,9 +405,13 @@ __device__ void md_candidate(const md_build_ctx &c, const md_list_ptrs &L, int f
 // wavefront per S, lanes over candidate J clusters, compacted with a
 // wavefront prefix count so the order is deterministic.
 template <int PASS>
-__global__ void __launch_bounds__(64) k_nb_list(md_build_ctx c, md_list_ptrs L, md_control *ctl, int forced) {
-  if (!md_build_gate(ctl, forced)) return;
-  int S = blockIdx.x;
+__global__ void __launch_bounds__(64) k_nb_list(const md_nb_args *__restrict__ T, md_bmap map) {
+  int S;
+  const md_nb_args &A = md_nb_entry(T, map, S);
+  md_control *ctl = A.ctl;
+  if (!ctl->rebuild_active) return;
+  const md_build_ctx &c = A.c;
+  const md_list_ptrs &L = A.L;
   int lane = threadIdx.x;
   if (S >= c.nsuper) return;
   float reach = __uint_as_float(ctl->cluster_reach_bits[0]) + __uint_as_float(ctl->cluster_reach_bits[1]);
@@ -415,10 +467,15 @@ __global__ void __launch_bounds__(64) k_nb_list(md_build_ctx c, md_list_ptrs L, 
 }
 
 // Starts of every super-cluster's entries and segments, and the totals.
-__global__ void __launch_bounds__(1024) k_nb_list_scan(int nsuper, md_list_ptrs L, int entry_capacity,
-                                                       int segment_capacity, md_control *ctl, int *totals,
-                                                       int forced) {
-  if (!md_build_gate(ctl, forced)) return;
+// one workgroup per replica
+__global__ void __launch_bounds__(1024) k_nb_list_scan(const md_nb_args *__restrict__ T, md_bmap map) {
+  int blk;
+  const md_nb_args &A = md_nb_entry(T, map, blk);
+  md_control *ctl = A.ctl;
+  if (!ctl->rebuild_active) return;
+  const md_list_ptrs &L = A.L;
+  const int nsuper = A.c.nsuper, entry_capacity = A.c.entry_capacity, segment_capacity = A.c.segment_capacity;
+  int *totals = A.totals;
   for (int s = threadIdx.x; s < nsuper; s += 1024) L.super_segs[s] = (L.super_count[s] + MD_SEGMENT - 1) / MD_SEGMENT;
   __syncthreads();
   md_block_scan(L.super_count, L.super_count, nsuper, nullptr);
@@ -433,11 +490,14 @@ __global__ void __launch_bounds__(1024) k_nb_list_scan(int nsuper, md_list_ptrs 
   }
 }
 
-__global__ void k_nb_segments(int nsuper, md_list_ptrs L, int entry_capacity, int segment_capacity,
-                              const md_control *ctl, int forced) {
-  if (!md_build_gate(ctl, forced)) return;
-  int S = blockIdx.x * blockDim.x + threadIdx.x;
-  if (S >= nsuper) return;
+__global__ void k_nb_segments(const md_nb_args *__restrict__ T, md_bmap map) {
+  int blk;
+  const md_nb_args &A = md_nb_entry(T, map, blk);
+  if (!A.ctl->rebuild_active) return;
+  const md_list_ptrs &L = A.L;
+  const int entry_capacity = A.c.entry_capacity, segment_capacity = A.c.segment_capacity;
+  int S = blk * blockDim.x + threadIdx.x;
+  if (S >= A.c.nsuper) return;
   int b = L.super_count[S], e = L.super_count[S + 1];
   int sg = L.super_segs[S];
   for (int at = b; at < e; at += MD_SEGMENT, ++sg) {
@@ -450,15 +510,19 @@ __global__ void k_nb_segments(int nsuper, md_list_ptrs L, int entry_capacity, in
 // each of S's 8 i-clusters a 64-bit mask of the pairs that interact.  Clears
 // padding, the lower triangle of a cluster with its own zero image, and every
 // excluded pair (the CSR holds both directions; nonbonded.ts:159-182).
-__global__ void __launch_bounds__(64) k_nb_masks(md_build_ctx c, md_list_ptrs L, const int *__restrict__ slot_atom,
-                                                 const int *__restrict__ atom_slot, const int *__restrict__ excl_start,
-                                                 const int *__restrict__ excl_list, const int *totals,
-                                                 const md_control *ctl, int forced) {
-  if (!md_build_gate(ctl, forced)) return;
-  int total = totals[0];
+__global__ void __launch_bounds__(64) k_nb_masks(const md_nb_args *__restrict__ T, md_bmap map) {
+  int blk;
+  const int r = md_locate(map, blockIdx.x, blk);
+  const md_nb_args &A = T[r];
+  if (!A.ctl->rebuild_active) return;
+  const int nblk = map.start[r + 1] - map.start[r];
+  const md_build_ctx &c = A.c;
+  const md_list_ptrs &L = A.L;
+  const int *slot_atom = A.slot_atom, *excl_start = A.excl_start, *excl_list = A.excl_list;
+  int total = A.totals[0];
   int lane = threadIdx.x, il = lane >> 3, jl = lane & 7;
   // one wave per entry, striding over the entries the list holds
-  for (int e = blockIdx.x; e < total; e += gridDim.x) {
+  for (int e = blk; e < total; e += nblk) {
     // which super-cluster owns entry e: binary search the starts
     int lo = 0, hi = c.nsuper;
     while (hi - lo > 1) {
@@ -490,11 +554,26 @@ __global__ void __launch_bounds__(64) k_nb_masks(md_build_ctx c, md_list_ptrs L,
 }
 
 // Per step: each slot's position as f32 in its super-cluster's frame.
-__global__ void k_nb_pack(int nslots, int np, const int *__restrict__ slot_atom, const int64_t *__restrict__ pos,
-                          const int64_t *__restrict__ slot_offset, const float4 *__restrict__ atom_param,
-                          float4 *slot_xyzq, md_control *ctl) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s == 0) ctl->segment_cursor = 0;  // the pair kernel that follows hands out segments from 0
+// The launch's first thread also lays out the pair kernel's work: the start of
+// each replica's list segments in one numbering, and the counter the pair
+// kernel's waves take segments from, reset to 0.
+__global__ void k_nb_pack(const md_nb_args *__restrict__ T, md_bmap map, int *seg_start, int *cursor) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    int run = 0;
+    for (int r = 0; r < map.R; ++r) {
+      seg_start[r] = run;
+      run += T[r].totals[1];
+    }
+    seg_start[map.R] = run;
+    *cursor = 0;
+  }
+  int blk;
+  const md_nb_args &A = md_nb_entry(T, map, blk);
+  const int nslots = A.c.nslots, np = A.c.np;
+  const int *slot_atom = A.slot_atom;
+  const int64_t *pos = A.pos, *slot_offset = A.slot_offset;
+  float4 *slot_xyzq = A.slot_xyzq;
+  int s = blk * blockDim.x + threadIdx.x;
   if (s >= nslots) return;
   int a = slot_atom[s];
   if (a < 0) {
@@ -505,30 +584,12 @@ __global__ void k_nb_pack(int nslots, int np, const int *__restrict__ slot_atom,
   float x = md_from_fixed(pos[a] - slot_offset[s]);
   float y = md_from_fixed(pos[np + a] - slot_offset[nslots + s]);
   float z = md_from_fixed(pos[2 * np + a] - slot_offset[2 * nslots + s]);
-  slot_xyzq[s] = make_float4(x, y, z, atom_param[a].x);
+  slot_xyzq[s] = make_float4(x, y, z, A.atom_param[a].x);
 }
 
 // ---------------------------------------------------------------------------
 // The pair kernel
 // ---------------------------------------------------------------------------
-
-struct md_pair_args {
-  const int4 *segments;
-  const int *entry_j;
-  const int *entry_imask;
-  const float4 *entry_shift;
-  const uint64_t *entry_excl;
-  const float4 *slot_xyzq;
-  const float2 *slot_lj;
-  const int *slot_atom;
-  const int *totals;
-  int64_t *force;
-  int64_t *energy;
-  int64_t *census;
-  int32_t *segment_cursor;
-  int np, nclusters;
-  float cutoff2, alpha, beta, exp_scale;
-};
 
 // one pair from lane's i atom to the entry's j atom; returns the scalar
 // F/r factor (so F_i = -fs * d) and accumulates the pair energy
@@ -650,27 +711,30 @@ __device__ __forceinline__ void md_pair_segment(const md_pair_args &A, int seg, 
 }
 
 template <bool ENERGY>
-__global__ void __launch_bounds__(64) k_nb_pairs(md_pair_args A) {
-  const int nseg = A.totals[1];
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_nb_pairs(const md_nb_args *__restrict__ T, md_bmap seg_map, int *cursor) {
+  const int nseg = seg_map.start[seg_map.R];
   const int lane = threadIdx.x, il = lane >> 3, jl = lane & 7;
-  // each segment's energy is rounded to Q32.32 on its own, so the sum does
-  // not depend on which wave took which segment
-  int64_t efix = 0;
-  int count = 0;
-  // a resident grid of waves takes segments from a counter until none are left
+  // A resident grid of waves takes segments from one counter until none are
+  // left, across every replica's list.  Each segment's energy is rounded to
+  // Q32.32 and added on its own, so the sums do not depend on which wave took
+  // which segment.
   for (;;) {
-    int seg = 0;
-    if (lane == 0) seg = atomicAdd(A.segment_cursor, 1);
-    seg = __shfl(seg, 0, 64);
-    if (seg >= nseg) break;
+    int g = 0;
+    if (lane == 0) g = atomicAdd(cursor, 1);
+    g = __shfl(g, 0, 64);
+    if (g >= nseg) break;
+    int seg;
+    const md_pair_args &A = T[md_locate(seg_map, g, seg)].P;
+    int64_t efix = 0;
+    int count = 0;
     md_pair_segment<ENERGY>(A, seg, lane, il, jl, efix, count);
-  }
-  if (ENERGY) {
-    int64_t ef = md_wave_sum_i64(efix);
-    int64_t cf = md_wave_sum_i64((int64_t)count);
-    if (lane == 0) {
-      md_atomic_add_i64(A.energy + MD_E_PAIR, ef);
-      md_atomic_add_i64(A.census, cf);
+    if (ENERGY) {
+      int64_t ef = md_wave_sum_i64(efix);
+      int64_t cf = md_wave_sum_i64((int64_t)count);
+      if (lane == 0) {
+        md_atomic_add_i64(A.energy + MD_E_PAIR, ef);
+        md_atomic_add_i64(A.census, cf);
+      }
     }
   }
 }
@@ -765,8 +829,6 @@ struct md_nb_buffers {
   int *totals = nullptr;
   int nsuper = 0;
   int entry_capacity = 0, segment_capacity = 0;
-  int pair_grid[2] = {1, 1};  // resident waves of k_nb_pairs<false>, <true>
-  int mask_grid = 1;
   float half_min_width = 0, list_cutoff = 0;
 };
 
@@ -843,21 +905,15 @@ void md_nb_setup(md_system *s) {
   double volume = std::fabs(s->box[0] * s->box[4] * s->box[8]);
   double density = s->n / volume;
   double ext = std::cbrt(MD_CLUSTER / std::max(density, 1.0));
-  double reach = rlist + 2.0 * ext;
-  double per_super = 8.0 * (4.0 / 3.0 * M_PI * reach * reach * reach) * density / MD_CLUSTER;
-  long cap = (long)(2.0 * B.nsuper * std::min(per_super, (double)nb.nclusters)) + 1024;
-  cap = std::min(cap, (long)B.nsuper * (long)nb.nclusters * 27 + 64);
-  B.entry_capacity = (int)cap;
+  // a super-cluster (a compact run of the cell walk, side ~ (64/density)^1/3)
+  // meets the j-clusters within the list radius plus a cluster's extent of its
+  // box: 1.5x that count, or every cluster at most (27 images of each)
+  double side = std::cbrt(MD_SUPER / std::max(density, 1.0));
+  double span = side + 2.0 * (rlist + ext);
+  double per_super = span * span * span * density / MD_CLUSTER;
+  long cap = (long)(1.5 * B.nsuper * std::min(per_super, 27.0 * nb.nclusters)) + 1024;
+  B.entry_capacity = (int)std::min(cap, (long)INT32_MAX / 16);
   B.segment_capacity = B.entry_capacity / MD_SEGMENT + B.nsuper + 64;
-  // the pair kernel keeps one resident grid busy from a segment counter; the
-  // exclusion masks stride a fixed grid over the entries
-  int cus = 1, per_cu[2] = {1, 1};
-  MD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));
-  MD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[0], k_nb_pairs<false>, 64, 0));
-  MD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[1], k_nb_pairs<true>, 64, 0));
-  for (int k = 0; k < 2; ++k)
-    B.pair_grid[k] = std::max(1, std::min(B.segment_capacity, std::max(1, cus) * std::max(1, per_cu[k])));
-  B.mask_grid = std::max(1, std::min(B.entry_capacity, std::max(1, cus) * 32));
   B.entry_j = md_alloc<int>(s, B.entry_capacity);
   B.entry_imask = md_alloc<int>(s, B.entry_capacity);
   B.entry_shift = md_alloc<float4>(s, B.entry_capacity);
@@ -894,14 +950,25 @@ static md_build_ctx build_ctx(md_system *s) {
   c.segment_capacity = B.segment_capacity;
   c.list_cutoff2 = s->nb.list_cutoff2;
   c.list_cutoff = B.list_cutoff;
-  c.half_min_width = B.half_min_width;
+  // half the narrowest perpendicular width of the current box
+  const double *b = s->box;
+  auto cross_norm = [](const double *u, const double *w) {
+    double x = u[1] * w[2] - u[2] * w[1], y = u[2] * w[0] - u[0] * w[2], z = u[0] * w[1] - u[1] * w[0];
+    return std::sqrt(x * x + y * y + z * z);
+  };
+  double v = b[0] * b[4] * b[8];
+  c.half_min_width =
+      (float)(0.5 * std::min(v / cross_norm(b + 3, b + 6), std::min(v / cross_norm(b + 6, b), v / cross_norm(b, b + 3))));
   c.box = s->dbox;
   return c;
 }
 
-static md_list_ptrs list_ptrs(md_system *s) {
+// one replica's table entry
+static md_nb_args nb_args(md_system *s) {
   auto &B = nbuf(s);
-  md_list_ptrs L;
+  md_nb_args a{};
+  a.c = build_ctx(s);
+  md_list_ptrs &L = a.L;
   L.cluster_center = B.cluster_center;
   L.cluster_half = s->cluster_half;
   L.super_origin = B.super_origin;
@@ -912,65 +979,138 @@ static md_list_ptrs list_ptrs(md_system *s) {
   L.entry_shift = B.entry_shift;
   L.segments = B.segments;
   L.entry_excl = B.entry_excl;
-  return L;
+  md_pair_args &P = a.P;
+  P.segments = B.segments;
+  P.entry_j = B.entry_j;
+  P.entry_imask = B.entry_imask;
+  P.entry_shift = B.entry_shift;
+  P.entry_excl = B.entry_excl;
+  P.slot_xyzq = s->slot_xyzq;
+  P.slot_lj = s->slot_lj;
+  P.slot_atom = s->slot_atom;
+  P.totals = B.totals;
+  P.force = s->force;
+  P.energy = s->energy;
+  P.census = &s->ctl->pair_census;
+  P.np = s->np;
+  P.nclusters = s->nb.nclusters;
+  P.cutoff2 = s->nb.cutoff2;
+  P.alpha = (float)s->alpha;
+  P.beta = (float)(2.0 * s->alpha / std::sqrt(M_PI));
+  P.exp_scale = (float)(-s->alpha * s->alpha / std::log(2.0));
+  a.pos = s->pos;
+  a.cell_rank = s->cell_rank;
+  a.cell_count = s->cell_count;
+  a.cell_start = s->cell_start;
+  a.cell_cursor = s->cell_cursor;
+  a.atom_cell = s->atom_cell;
+  a.cell_member = s->cell_member;
+  a.slot_atom = s->slot_atom;
+  a.atom_slot = s->atom_slot;
+  a.folded = s->atom_folded;
+  a.build_pos = s->build_pos;
+  a.slot_offset = s->slot_offset;
+  a.cluster_center = B.cluster_center;
+  a.super_origin = B.super_origin;
+  a.slot_lj = s->slot_lj;
+  a.slot_xyzq = s->slot_xyzq;
+  a.cluster_half = s->cluster_half;
+  a.atom_param = s->atom_param;
+  a.excl_start = s->excl_start_d;
+  a.excl_list = s->excl_list_d;
+  a.totals = B.totals;
+  a.ctl = s->ctl;
+  return a;
 }
 
-void md_nb_build(md_system *s, bool forced) {
-  auto &B = nbuf(s);
-  md_build_ctx c = build_ctx(s);
-  md_list_ptrs L = list_ptrs(s);
-  int f = forced ? 1 : 0;
-  hipStream_t st = s->stream;
-  int n = s->n;
-  int g = (n + 255) / 256;
-  k_nb_assign<<<g, 256, 0, st>>>(c, s->pos, s->cell_rank, s->cell_count, s->atom_cell, s->atom_folded,
-                                 s->build_pos, s->ctl, f);
-  k_nb_scan_cells<<<1, 1024, 0, st>>>(c.ncells, s->cell_count, s->cell_start, s->cell_cursor, s->ctl, f);
-  k_nb_scatter<<<g, 256, 0, st>>>(n, s->atom_cell, s->cell_start, s->cell_cursor, s->cell_member, s->ctl, f);
-  k_nb_rank<<<g, 256, 0, st>>>(n, s->atom_cell, s->cell_start, s->cell_member, s->slot_atom, s->atom_slot, s->ctl, f);
-  int gc = (c.nclusters + 63) / 64;
-  k_nb_clusters<<<gc, 64, 0, st>>>(c, s->slot_atom, s->atom_folded, s->build_pos, s->atom_param, B.cluster_center,
-                                   s->cluster_half, B.super_origin, s->slot_offset, s->slot_lj, s->ctl, f);
-  k_nb_list<0><<<B.nsuper, 64, 0, st>>>(c, L, s->ctl, f);
-  k_nb_list_scan<<<1, 1024, 0, st>>>(B.nsuper, L, B.entry_capacity, B.segment_capacity, s->ctl, B.totals, f);
-  k_nb_list<1><<<B.nsuper, 64, 0, st>>>(c, L, s->ctl, f);
-  k_nb_segments<<<(B.nsuper + 63) / 64, 64, 0, st>>>(B.nsuper, L, B.entry_capacity, B.segment_capacity, s->ctl, f);
-  k_nb_masks<<<B.mask_grid, 64, 0, st>>>(c, L, s->slot_atom, s->atom_slot, s->excl_start_d, s->excl_list_d,
-                                              B.totals, s->ctl, f);
+// a group's launch tables for the nonbonded kernels
+struct md_nb_group {
+  md_nb_args *table = nullptr;  // [R], device
+  md_launch atoms, scan, clusters, supers, segments, masks, slots;
+  md_launch seg;                // the pair kernel's segment numbering
+  int *cursor = nullptr;
+  int pair_grid[2] = {1, 1};
+};
+
+static md_nb_group &nbg(md_group *g) { return *reinterpret_cast<md_nb_group *>(g->nb); }
+
+void md_nb_group_setup(md_group *g) {
+  auto *G = new md_nb_group();
+  g->nb = G;
+  const int R = (int)g->reps.size();
+  std::vector<int> atoms, one, clusters, supers, segments, masks, slots;
+  for (md_system *s : g->reps) {
+    auto &B = nbuf(s);
+    atoms.push_back((s->n + 255) / 256);
+    one.push_back(1);
+    clusters.push_back((s->nb.nclusters + 63) / 64);
+    supers.push_back(B.nsuper);
+    segments.push_back((B.nsuper + 63) / 64);
+    masks.push_back(std::max(1, std::min(B.entry_capacity, B.nsuper * 32)));
+    slots.push_back((s->nb.nslots + 255) / 256);
+  }
+  G->atoms = md_make_launch(g, atoms);
+  G->scan = md_make_launch(g, one);
+  G->clusters = md_make_launch(g, clusters);
+  G->supers = md_make_launch(g, supers);
+  G->segments = md_make_launch(g, segments);
+  G->masks = md_make_launch(g, masks);
+  G->slots = md_make_launch(g, slots);
+  G->seg.map.start = md_group_alloc<int>(g, R + 1);
+  G->seg.map.R = R;
+  G->cursor = md_group_alloc<int>(g, 1);
+  G->table = md_group_alloc<md_nb_args>(g, R);
+  // the pair kernel keeps one resident grid busy from the segment counter
+  int cus = 1, per_cu[2] = {1, 1};
+  MD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
+  MD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[0], k_nb_pairs<false>, 64, 0));
+  MD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[1], k_nb_pairs<true>, 64, 0));
+  long capacity = 0;
+  for (md_system *s : g->reps) capacity += nbuf(s).segment_capacity;
+  for (int k = 0; k < 2; ++k)
+    G->pair_grid[k] = (int)std::max(1L, std::min(capacity, (long)std::max(1, cus) * std::max(1, per_cu[k])));
+  md_nb_group_refresh(g);
+}
+
+void md_nb_group_refresh(md_group *g) {
+  std::vector<md_nb_args> host;
+  for (md_system *s : g->reps) host.push_back(nb_args(s));
+  MD_HIP(hipMemcpyAsync(nbg(g).table, host.data(), sizeof(md_nb_args) * host.size(), hipMemcpyHostToDevice,
+                        g->stream));
+  MD_HIP(hipStreamSynchronize(g->stream));
+}
+
+void md_nb_group_free(md_group *g) {
+  delete &nbg(g);
+  g->nb = nullptr;
+}
+
+void md_nb_build(md_group *g) {
+  auto &G = nbg(g);
+  hipStream_t st = g->stream;
+  const md_nb_args *T = G.table;
+  k_nb_assign<<<G.atoms.total, 256, 0, st>>>(T, G.atoms.map);
+  k_nb_scan_cells<<<G.scan.total, 1024, 0, st>>>(T, G.scan.map);
+  k_nb_scatter<<<G.atoms.total, 256, 0, st>>>(T, G.atoms.map);
+  k_nb_rank<<<G.atoms.total, 256, 0, st>>>(T, G.atoms.map);
+  k_nb_clusters<<<G.clusters.total, 64, 0, st>>>(T, G.clusters.map);
+  k_nb_list<0><<<G.supers.total, 64, 0, st>>>(T, G.supers.map);
+  k_nb_list_scan<<<G.scan.total, 1024, 0, st>>>(T, G.scan.map);
+  k_nb_list<1><<<G.supers.total, 64, 0, st>>>(T, G.supers.map);
+  k_nb_segments<<<G.segments.total, 64, 0, st>>>(T, G.segments.map);
+  k_nb_masks<<<G.masks.total, 64, 0, st>>>(T, G.masks.map);
   MD_HIP(hipGetLastError());
 }
 
-void md_nb_pairs(md_system *s, bool energy, bool census) {
-  auto &B = nbuf(s);
-  hipStream_t st = s->stream;
-  int gs = (s->nb.nslots + 255) / 256;
-  k_nb_pack<<<gs, 256, 0, st>>>(s->nb.nslots, s->np, s->slot_atom, s->pos, s->slot_offset, s->atom_param,
-                                s->slot_xyzq, s->ctl);
-  md_mark(s, MD_PHASE_PAIRS);
-  md_pair_args A;
-  A.segments = B.segments;
-  A.entry_j = B.entry_j;
-  A.entry_imask = B.entry_imask;
-  A.entry_shift = B.entry_shift;
-  A.entry_excl = B.entry_excl;
-  A.slot_xyzq = s->slot_xyzq;
-  A.slot_lj = s->slot_lj;
-  A.slot_atom = s->slot_atom;
-  A.totals = B.totals;
-  A.force = s->force;
-  A.energy = s->energy;
-  A.census = &s->ctl->pair_census;
-  A.segment_cursor = &s->ctl->segment_cursor;
-  A.np = s->np;
-  A.nclusters = s->nb.nclusters;
-  A.cutoff2 = s->nb.cutoff2;
-  A.alpha = (float)s->alpha;
-  A.beta = (float)(2.0 * s->alpha / std::sqrt(M_PI));
-  A.exp_scale = (float)(-s->alpha * s->alpha / std::log(2.0));
-  if (energy || census)
-    k_nb_pairs<true><<<B.pair_grid[1], 64, 0, st>>>(A);
+void md_nb_pairs(md_group *g, bool energy) {
+  auto &G = nbg(g);
+  hipStream_t st = g->stream;
+  k_nb_pack<<<G.slots.total, 256, 0, st>>>(G.table, G.slots.map, const_cast<int *>(G.seg.map.start), G.cursor);
+  md_mark(g, MD_PHASE_PAIRS);
+  if (energy)
+    k_nb_pairs<true><<<G.pair_grid[1], 64, 0, st>>>(G.table, G.seg.map, G.cursor);
   else
-    k_nb_pairs<false><<<B.pair_grid[0], 64, 0, st>>>(A);
+    k_nb_pairs<false><<<G.pair_grid[0], 64, 0, st>>>(G.table, G.seg.map, G.cursor);
   MD_HIP(hipGetLastError());
 }
 

@@ -42,6 +42,23 @@ struct md_pme_args {
   float exp_scale;   // -1/(4 alpha^2)
 };
 
+// one replica's table entry
+struct md_pme_table {
+  md_pme_args P;
+  int n, np;
+  const int64_t *pos;
+  const float4 *param;
+  int64_t *grid;
+  float2 *cgrid;
+  float *phi;
+  int64_t *force;
+  int64_t *energy;
+};
+
+__device__ __forceinline__ const md_pme_table &md_pme_entry(const md_pme_table *T, const md_bmap &map, int &blk) {
+  return T[md_locate(map, blockIdx.x, blk)];
+}
+
 // ---- B-splines (src/reference/forces/bspline.ts:55-91, order 5) ------------
 
 __device__ __forceinline__ void md_bspline5(float w, float *th, float *dth) {
@@ -93,16 +110,19 @@ __device__ __forceinline__ int md_wrap(int i, int n) {
 }
 
 // one thread per (atom, x-plane of the stencil): 25 integer atomics each
-__global__ void k_pme_spread(int n, int np, const int64_t *__restrict__ pos, const float4 *__restrict__ param,
-                             md_pme_args P, int64_t *grid) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_pme_spread(const md_pme_table *__restrict__ T, md_bmap map) {
+  int blk;
+  const md_pme_table &E = md_pme_entry(T, map, blk);
+  const md_pme_args &P = E.P;
+  int64_t *grid = E.grid;
+  int t = blk * blockDim.x + threadIdx.x;
   int i = t / MD_PME_ORDER, a = t - i * MD_PME_ORDER;
-  if (i >= n) return;
-  float q = param[i].w;
+  if (i >= E.n) return;
+  float q = E.param[i].w;
   if (q == 0.0f) return;
   int base[3];
   float w[3];
-  md_mesh_coord(pos, np, i, P.inv, P.nx, P.ny, P.nz, base, w);
+  md_mesh_coord(E.pos, E.np, i, P.inv, P.nx, P.ny, P.nz, base, w);
   float tx[5], ty[5], tz[5];
   md_bspline5(w[0], tx, nullptr);
   md_bspline5(w[1], ty, nullptr);
@@ -191,10 +211,15 @@ __device__ float2 *md_lds_fft(float2 *a, float2 *b, int nlines, const md_fft_pla
 
 // z pass, forward: int64 rows -> f32 -> complex FFT -> first nz/2+1 values.
 // Clears the integer grid behind it.  Block = PME_BATCH rows.
-__global__ void __launch_bounds__(256) k_pme_z_forward(md_pme_args P, int64_t *grid, float2 *cgrid) {
+__global__ void __launch_bounds__(256) k_pme_z_forward(const md_pme_table *__restrict__ T, md_bmap map) {
   extern __shared__ float2 lds[];
+  int blk;
+  const md_pme_table &E = md_pme_entry(T, map, blk);
+  const md_pme_args &P = E.P;
+  int64_t *grid = E.grid;
+  float2 *cgrid = E.cgrid;
   int rows = P.nx * P.ny;
-  int r0 = blockIdx.x * PME_BATCH;
+  int r0 = blk * PME_BATCH;
   int nl = min(PME_BATCH, rows - r0);
   float2 *a = lds, *b = lds + PME_BATCH * P.nz;
   for (int idx = threadIdx.x; idx < nl * P.nz; idx += blockDim.x) {
@@ -214,11 +239,15 @@ __global__ void __launch_bounds__(256) k_pme_z_forward(md_pme_args P, int64_t *g
 
 // y pass: lines along y at fixed (x, kz); a block takes PME_BATCH
 // consecutive kz at one x, so global reads are contiguous runs.
-__global__ void __launch_bounds__(256) k_pme_y(md_pme_args P, float2 *cgrid, float sign) {
+__global__ void __launch_bounds__(256) k_pme_y(const md_pme_table *__restrict__ T, md_bmap map, float sign) {
   extern __shared__ float2 lds[];
+  int blk;
+  const md_pme_table &E = md_pme_entry(T, map, blk);
+  const md_pme_args &P = E.P;
+  float2 *cgrid = E.cgrid;
   int kzb = (P.nzc + PME_BATCH - 1) / PME_BATCH;
-  int x = blockIdx.x / kzb;
-  int z0 = (blockIdx.x - x * kzb) * PME_BATCH;
+  int x = blk / kzb;
+  int z0 = (blk - x * kzb) * PME_BATCH;
   int nl = min(PME_BATCH, P.nzc - z0);
   float2 *a = lds, *b = lds + PME_BATCH * P.ny;
   for (int idx = threadIdx.x; idx < nl * P.ny; idx += blockDim.x) {
@@ -236,11 +265,16 @@ __global__ void __launch_bounds__(256) k_pme_y(md_pme_args P, float2 *cgrid, flo
 // x pass with the convolution: forward transform along x, multiply by the
 // Ewald influence function (accumulating the energy), inverse transform.
 template <bool ENERGY>
-__global__ void __launch_bounds__(256) k_pme_x_convolve(md_pme_args P, float2 *cgrid, int64_t *energy) {
+__global__ void __launch_bounds__(256) k_pme_x_convolve(const md_pme_table *__restrict__ T, md_bmap map) {
   extern __shared__ float2 lds[];
+  int blk;
+  const md_pme_table &E = md_pme_entry(T, map, blk);
+  const md_pme_args &P = E.P;
+  float2 *cgrid = E.cgrid;
+  int64_t *energy = E.energy;
   int kzb = (P.nzc + PME_BATCH - 1) / PME_BATCH;
-  int y = blockIdx.x / kzb;
-  int z0 = (blockIdx.x - y * kzb) * PME_BATCH;
+  int y = blk / kzb;
+  int z0 = (blk - y * kzb) * PME_BATCH;
   int nl = min(PME_BATCH, P.nzc - z0);
   float2 *a = lds, *b = lds + PME_BATCH * P.nx;
   size_t xstride = (size_t)P.ny * P.nzc;
@@ -291,10 +325,15 @@ __global__ void __launch_bounds__(256) k_pme_x_convolve(md_pme_args P, float2 *c
 }
 
 // z pass, inverse: Hermitian half line -> full line -> inverse FFT -> real.
-__global__ void __launch_bounds__(256) k_pme_z_inverse(md_pme_args P, const float2 *cgrid, float *phi) {
+__global__ void __launch_bounds__(256) k_pme_z_inverse(const md_pme_table *__restrict__ T, md_bmap map) {
   extern __shared__ float2 lds[];
+  int blk;
+  const md_pme_table &E = md_pme_entry(T, map, blk);
+  const md_pme_args &P = E.P;
+  const float2 *cgrid = E.cgrid;
+  float *phi = E.phi;
   int rows = P.nx * P.ny;
-  int r0 = blockIdx.x * PME_BATCH;
+  int r0 = blk * PME_BATCH;
   int nl = min(PME_BATCH, rows - r0);
   float2 *a = lds, *b = lds + PME_BATCH * P.nz;
   for (int idx = threadIdx.x; idx < nl * P.nz; idx += blockDim.x) {
@@ -319,15 +358,20 @@ __global__ void __launch_bounds__(256) k_pme_z_inverse(md_pme_args P, const floa
 }
 
 // interpolate dE/dQ back onto the atoms: one thread per atom
-__global__ void k_pme_gather(int n, int np, const int64_t *__restrict__ pos, const float4 *__restrict__ param,
-                             md_pme_args P, const float *__restrict__ phi, int64_t *force) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float q = param[i].w;
+__global__ void k_pme_gather(const md_pme_table *__restrict__ T, md_bmap map) {
+  int blk;
+  const md_pme_table &E = md_pme_entry(T, map, blk);
+  const md_pme_args &P = E.P;
+  const float *phi = E.phi;
+  int64_t *force = E.force;
+  const int np = E.np;
+  int i = blk * blockDim.x + threadIdx.x;
+  if (i >= E.n) return;
+  float q = E.param[i].w;
   if (q == 0.0f) return;
   int base[3];
   float w[3];
-  md_mesh_coord(pos, np, i, P.inv, P.nx, P.ny, P.nz, base, w);
+  md_mesh_coord(E.pos, np, i, P.inv, P.nx, P.ny, P.nz, base, w);
   float tx[5], ty[5], tz[5], dx[5], dy[5], dz[5];
   md_bspline5(w[0], tx, dx);
   md_bspline5(w[1], ty, dy);
@@ -447,13 +491,6 @@ void md_pme_setup(md_system *s) {
   s->pme_cgrid = md_alloc<float2>(s, (size_t)nx * ny * s->pme_nzc);
   s->pme_phi = md_alloc<float>(s, g);
   MD_HIP(hipStreamSynchronize(s->stream));
-  int maxn = std::max(nx, std::max(ny, nz));
-  size_t lds = sizeof(float2) * 2 * PME_BATCH * maxn;
-  if (lds > 64 * 1024) {
-    for (auto *k : {(const void *)k_pme_z_forward, (const void *)k_pme_y, (const void *)k_pme_x_convolve<true>,
-                    (const void *)k_pme_x_convolve<false>, (const void *)k_pme_z_inverse})
-      MD_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  }
 }
 
 void md_pme_free(md_system *s) {
@@ -485,26 +522,95 @@ static md_pme_args pme_args(md_system *s) {
   return P;
 }
 
-void md_pme(md_system *s, bool energy) {
-  md_pme_args P = pme_args(s);
-  hipStream_t st = s->stream;
-  int n = s->n;
-  k_pme_spread<<<(n * MD_PME_ORDER + 255) / 256, 256, 0, st>>>(n, s->np, s->pos, s->atom_param, P, s->pme_grid);
-  int rows = P.nx * P.ny;
-  size_t lz = sizeof(float2) * 2 * PME_BATCH * P.nz;
-  size_t ly = sizeof(float2) * 2 * PME_BATCH * P.ny;
-  size_t lx = sizeof(float2) * 2 * PME_BATCH * P.nx;
-  int kzb = (P.nzc + PME_BATCH - 1) / PME_BATCH;
-  md_mark(s, MD_PHASE_FFT);
-  k_pme_z_forward<<<(rows + PME_BATCH - 1) / PME_BATCH, 256, lz, st>>>(P, s->pme_grid, s->pme_cgrid);
-  k_pme_y<<<P.nx * kzb, 256, ly, st>>>(P, s->pme_cgrid, -1.0f);
+static md_pme_table pme_table(md_system *s) {
+  md_pme_table E;
+  E.P = pme_args(s);
+  E.n = s->n;
+  E.np = s->np;
+  E.pos = s->pos;
+  E.param = s->atom_param;
+  E.grid = s->pme_grid;
+  E.cgrid = s->pme_cgrid;
+  E.phi = s->pme_phi;
+  E.force = s->force;
+  E.energy = s->energy;
+  return E;
+}
+
+struct md_pme_group {
+  md_pme_table *table = nullptr;
+  md_launch spread, zrows, ylines, xlines, gather;
+  size_t lz = 0, ly = 0, lx = 0;  // dynamic LDS of the three passes
+};
+
+static md_pme_group &pg(md_group *g) { return *reinterpret_cast<md_pme_group *>(g->pme); }
+
+void md_pme_group_setup(md_group *g) {
+  auto *G = new md_pme_group();
+  g->pme = G;
+  std::vector<int> spread, zrows, ylines, xlines, gather;
+  int mx = 0, my = 0, mz = 0;
+  for (md_system *s : g->reps) {
+    bool on = s->has_nonbonded;
+    int nx = s->pme_nx, ny = s->pme_ny, nz = s->pme_nz, kzb = (s->pme_nzc + PME_BATCH - 1) / PME_BATCH;
+    spread.push_back(on ? (s->n * MD_PME_ORDER + 255) / 256 : 0);
+    zrows.push_back(on ? (nx * ny + PME_BATCH - 1) / PME_BATCH : 0);
+    ylines.push_back(on ? nx * kzb : 0);
+    xlines.push_back(on ? ny * kzb : 0);
+    gather.push_back(on ? (s->n + 255) / 256 : 0);
+    if (on) {
+      mx = std::max(mx, nx);
+      my = std::max(my, ny);
+      mz = std::max(mz, nz);
+    }
+  }
+  G->spread = md_make_launch(g, spread);
+  G->zrows = md_make_launch(g, zrows);
+  G->ylines = md_make_launch(g, ylines);
+  G->xlines = md_make_launch(g, xlines);
+  G->gather = md_make_launch(g, gather);
+  G->lz = sizeof(float2) * 2 * PME_BATCH * mz;
+  G->ly = sizeof(float2) * 2 * PME_BATCH * my;
+  G->lx = sizeof(float2) * 2 * PME_BATCH * mx;
+  size_t lds = std::max(G->lz, std::max(G->ly, G->lx));
+  if (lds > 64 * 1024) {
+    for (auto *k : {(const void *)k_pme_z_forward, (const void *)k_pme_y, (const void *)k_pme_x_convolve<true>,
+                    (const void *)k_pme_x_convolve<false>, (const void *)k_pme_z_inverse})
+      MD_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  }
+  G->table = md_group_alloc<md_pme_table>(g, g->reps.size());
+  md_pme_group_refresh(g);
+}
+
+void md_pme_group_refresh(md_group *g) {
+  std::vector<md_pme_table> host;
+  for (md_system *s : g->reps) host.push_back(s->has_nonbonded ? pme_table(s) : md_pme_table{});
+  MD_HIP(hipMemcpyAsync(pg(g).table, host.data(), sizeof(md_pme_table) * host.size(), hipMemcpyHostToDevice,
+                        g->stream));
+  MD_HIP(hipStreamSynchronize(g->stream));
+}
+
+void md_pme_group_free(md_group *g) {
+  delete &pg(g);
+  g->pme = nullptr;
+}
+
+void md_pme(md_group *g, bool energy) {
+  auto &G = pg(g);
+  if (G.spread.total == 0) return;
+  hipStream_t st = g->stream;
+  const md_pme_table *T = G.table;
+  k_pme_spread<<<G.spread.total, 256, 0, st>>>(T, G.spread.map);
+  md_mark(g, MD_PHASE_FFT);
+  k_pme_z_forward<<<G.zrows.total, 256, G.lz, st>>>(T, G.zrows.map);
+  k_pme_y<<<G.ylines.total, 256, G.ly, st>>>(T, G.ylines.map, -1.0f);
   if (energy)
-    k_pme_x_convolve<true><<<P.ny * kzb, 256, lx, st>>>(P, s->pme_cgrid, s->energy);
+    k_pme_x_convolve<true><<<G.xlines.total, 256, G.lx, st>>>(T, G.xlines.map);
   else
-    k_pme_x_convolve<false><<<P.ny * kzb, 256, lx, st>>>(P, s->pme_cgrid, s->energy);
-  k_pme_y<<<P.nx * kzb, 256, ly, st>>>(P, s->pme_cgrid, 1.0f);
-  k_pme_z_inverse<<<(rows + PME_BATCH - 1) / PME_BATCH, 256, lz, st>>>(P, s->pme_cgrid, s->pme_phi);
-  md_mark(s, MD_PHASE_GATHER);
-  k_pme_gather<<<(n + 255) / 256, 256, 0, st>>>(n, s->np, s->pos, s->atom_param, P, s->pme_phi, s->force);
+    k_pme_x_convolve<false><<<G.xlines.total, 256, G.lx, st>>>(T, G.xlines.map);
+  k_pme_y<<<G.ylines.total, 256, G.ly, st>>>(T, G.ylines.map, 1.0f);
+  k_pme_z_inverse<<<G.zrows.total, 256, G.lz, st>>>(T, G.zrows.map);
+  md_mark(g, MD_PHASE_GATHER);
+  k_pme_gather<<<G.gather.total, 256, 0, st>>>(T, G.gather.map);
   MD_HIP(hipGetLastError());
 }

@@ -296,6 +296,36 @@ class HipEngine:
         return 0
 
 
+class HipBatch:
+    """Several ``HipEngine`` replicas advanced together: every phase of a
+    step is one kernel launch over all of them and strides are captured as
+    HIP graphs.  Each replica keeps its own options (temperature, seed, ...)
+    and all of its own methods between batch runs; the batch borrows the
+    engines, which must outlive it.  No counterpart upstream (one system per
+    device there); replicas under a Monte Carlo barostat are refused, since
+    its moves are host-driven per system."""
+
+    def __init__(self, engines: Sequence[HipEngine]):
+        self.engines = list(engines)
+        for e in self.engines:
+            if e.barostat is not None and e.barostat.frequency > 0:
+                raise ValueError("a batched replica cannot carry a Monte Carlo barostat")
+        self._native = native.NativeBatch([e._native for e in self.engines])
+
+    def run(self, steps: int) -> None:
+        self._native.run(int(steps))
+
+    def profileStep(self, repeats: int = 10) -> dict:
+        return self._native.profile_steps(repeats)
+
+    @property
+    def particleCount(self) -> int:
+        return sum(e.particleCount for e in self.engines)
+
+    def destroy(self) -> None:
+        self._native.close()
+
+
 def rattle_host(system: OpenMMSystem, pos: np.ndarray, vel: np.ndarray, inv: np.ndarray, tol: float,
                 max_iterations: int = 150) -> np.ndarray:
     """Jacobi RATTLE over every constraint (src/reference/constraints.ts:195-243)."""

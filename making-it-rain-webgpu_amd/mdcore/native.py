@@ -139,6 +139,11 @@ def library():
         L.md_constrain_velocities.argtypes = [C.c_void_p, C.c_double]
         L.md_get_mesh.argtypes = [C.c_void_p, _ip]
         L.md_probe_cell_order.argtypes = [_ip, _ip]
+        L.md_batch_create.argtypes = [P(C.c_void_p), C.c_int32, P(C.c_void_p)]
+        L.md_batch_destroy.argtypes = [C.c_void_p]
+        L.md_batch_destroy.restype = None
+        L.md_batch_run.argtypes = [C.c_void_p, C.c_int64]
+        L.md_batch_profile_steps.argtypes = [C.c_void_p, C.c_int32, _dp]
         _lib = L
     return _lib
 
@@ -339,6 +344,38 @@ class NativeSystem:
 
     def constrain_velocities(self, tolerance: float):
         check(library().md_constrain_velocities(self.handle, float(tolerance)))
+
+
+class NativeBatch:
+    """An ``md_batch*``: several ``NativeSystem`` replicas advanced together,
+    one kernel launch per phase for all of them.  Holds references to its
+    systems so they outlive it."""
+
+    def __init__(self, systems):
+        self.systems = list(systems)
+        arr = (C.c_void_p * len(self.systems))(*[s.handle.value for s in self.systems])
+        h = C.c_void_p()
+        check(library().md_batch_create(arr, len(self.systems), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            library().md_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, steps: int):
+        check(library().md_batch_run(self._h, int(steps)))
+
+    def profile_steps(self, steps: int) -> dict:
+        ms = np.zeros(PHASE_COUNT)
+        check(library().md_batch_profile_steps(self._h, int(steps), _d(ms)))
+        return {library().md_phase_name(k).decode(): float(ms[k]) for k in range(PHASE_COUNT)}
 
 
 PHASE_COUNT = 9

@@ -428,4 +428,42 @@ def config(index: int, seed: Optional[int] = None) -> Built:
         return build(1000, 33000, 0, "dodecahedron", s, restraint_k=1000.0, barostat=True)
     if index == 4:
         return build(20000, 76667, 0, "cube", s)
-    raise ValueError(f"no configuration {index}")
+    raise ValueError(f"no configuration {index} (the 64-replica ensemble is `ensemble()`)")
+
+
+@dataclass
+class Replica:
+    built: Built
+    seed: int
+    temperature: float
+
+
+def ensemble_plan(count: int = 64, seed: Optional[int] = None, low: int = 2000, high: int = 30000):
+    """Sizes, shapes, seeds and temperatures of the batched ensemble: atom
+    counts log-uniform on [low, high], cubic and octahedral cells
+    alternating, per-replica seeds, temperatures uniform on [290, 320] K."""
+    s = 0x20261202 + 3 if seed is None else seed
+    rng = np.random.default_rng(s)
+    sizes = np.exp(rng.uniform(np.log(low), np.log(high), count))
+    temps = rng.uniform(290.0, 320.0, count)
+    seeds = rng.integers(1, 2 ** 31 - 1, count)
+    plan = []
+    for k in range(count):
+        n = int(sizes[k])
+        chain = max(40, int(round(0.1 * n)))
+        ions = 2 * max(1, n // 10000)
+        waters = max(10, (n - chain - ions) // 3)
+        plan.append(dict(n_chain=chain, n_waters=waters, n_ions=ions,
+                         shape="cube" if k % 2 == 0 else "octahedron",
+                         seed=int(seeds[k]), temperature=float(temps[k])))
+    return plan
+
+
+def ensemble(count: int = 64, seed: Optional[int] = None, low: int = 2000, high: int = 30000):
+    """The batched ensemble: ``count`` independent solvated chains (see
+    ``ensemble_plan``), each with its own seed and temperature."""
+    out = []
+    for p in ensemble_plan(count, seed, low, high):
+        b = build(p["n_chain"], p["n_waters"], p["n_ions"], p["shape"], p["seed"])
+        out.append(Replica(b, p["seed"], p["temperature"]))
+    return out
