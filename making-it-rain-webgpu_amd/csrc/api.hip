@@ -599,8 +599,8 @@ static void check_control(md_system *s) {
   MD_HIP(hipMemcpy(&c, s->ctl, sizeof c, hipMemcpyDeviceToHost));
   if (c.overflow == 2)
     throw md_error(MD_ERR_CUTOFF_TOO_LARGE,
-                   "the list radius plus the cluster extents exceeds the narrowest box width: the box is too "
-                   "small for this cutoff");
+                   "the list radius plus the cluster extents reaches past four periodic images along an axis: "
+                   "the box is too small for this cutoff");
   if (c.overflow)
     throw md_error(MD_ERR_LIST_OVERFLOW,
                    "the cluster-pair list outgrew its capacity (the system compressed far beyond its starting "
@@ -1106,6 +1106,11 @@ md_err md_warmstart_record(md_warmstart *ws, md_system *s, int64_t steps) {
   s->rec_t = ts;
   s->rec_first_step = s->steps_done;
   s->rec_cap = fit * s->n_clusters_c;
+  // slots the run leaves unwritten (cluster-steps SHAKE did not converge on)
+  // read as clusters with no constraint: no loss, no gradient
+  MD_HIP(hipSetDevice(s->device));
+  MD_HIP(hipMemsetAsync(xs, 0, sizeof(float) * MD_WS_F * s->rec_cap, s->stream));
+  MD_HIP(hipMemsetAsync(ts, 0, sizeof(float) * MD_WS_MAXC * s->rec_cap, s->stream));
   try {
     refresh_all(s);
     run_steps(s, fit);

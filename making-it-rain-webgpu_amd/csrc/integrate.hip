@@ -248,19 +248,46 @@ __device__ void md_shake(const md_int_args &A, md_item<MAXA> &it, int nc, const 
   int applied = 0;
   for (int c = 0; c < nc; ++c) delta[c] = 0.0;
   if (lam0) {
-    // the predicted multipliers (in units of each constraint's reduced-mass
+    // The predicted multipliers (in units of each constraint's reduced-mass
     // factor mu), applied as one update; the passes below correct whatever
-    // the prediction missed
-    for (int c = 0; c < nc && c < MD_WS_MAXC; ++c) delta[c] = mu[c] * (double)lam0[c];
-    for (int c = 0; c < nc; ++c) {
-      int li = cons[c].x, lj = cons[c].y;
-      for (int d = 0; d < 3; ++d) {
-        double dr = rv[c][d] * delta[c];
-        t[li][d] += dr * it.im[li];
-        t[lj][d] -= dr * it.im[lj];
-        moved[li][d] += dr * it.im[li];
-        moved[lj][d] -= dr * it.im[lj];
+    // the prediction missed.  A prediction that leaves the constraints
+    // further from their lengths than the unconstrained trial positions is
+    // dropped, so a poor network never costs SHAKE more than a cold start.
+    auto worst = [&](const double (*p)[3]) {
+      double w = 0.0;
+      for (int c = 0; c < nc; ++c) {
+        int li = cons[c].x, lj = cons[c].y;
+        double x = p[li][0] - p[lj][0], y = p[li][1] - p[lj][1], z = p[li][2] - p[lj][2];
+        w = fmax(w, fabs((x * x + y * y + z * z) / t2[c] - 1.0));
       }
+      return w;
+    };
+    double tw[MAXA][3];
+    for (int a = 0; a < it.n; ++a)
+      for (int d = 0; d < 3; ++d) tw[a][d] = t[a][d];
+    bool finite = true;
+    for (int c = 0; c < nc && c < MD_WS_MAXC; ++c) {
+      delta[c] = mu[c] * (double)lam0[c];
+      finite &= isfinite(delta[c]);
+    }
+    if (finite) {
+      for (int c = 0; c < nc; ++c) {
+        int li = cons[c].x, lj = cons[c].y;
+        for (int d = 0; d < 3; ++d) {
+          double dr = rv[c][d] * delta[c];
+          tw[li][d] += dr * it.im[li];
+          tw[lj][d] -= dr * it.im[lj];
+        }
+      }
+    }
+    if (finite && worst(tw) < worst(t)) {
+      for (int a = 0; a < it.n; ++a)
+        for (int d = 0; d < 3; ++d) {
+          moved[a][d] = tw[a][d] - t[a][d];
+          t[a][d] = tw[a][d];
+        }
+    } else {
+      for (int c = 0; c < nc; ++c) delta[c] = 0.0;
     }
   }
   double total[MD_MAX_CLUSTER_CONSTRAINTS];
@@ -559,7 +586,9 @@ __global__ void __launch_bounds__(256) k_integrate(const md_int_args *__restrict
       float f[MD_WS_F];
       if (MODE == 1 && covered) md_ws_features(it, nc, cons, f);
       md_shake(A, it, nc, cons, warm ? lam0 : nullptr, MODE == 1 ? total : nullptr, &passes);
-      if (MODE == 1 && covered) {
+      if (MODE == 1 && covered && passes < MD_MAX_CONSTRAINT_ITERATIONS) {
+        // a cluster-step SHAKE did not converge teaches nothing: its slot
+        // keeps the zero features (no constraint present) it was cleared to
         int64_t k = (A.ctl->step_current - A.rec_first_step) * (int64_t)A.n_ccl + t;
         if (k >= 0 && k < A.rec_cap) {
           for (int q = 0; q < MD_WS_F; ++q) A.rec_x[k * MD_WS_F + q] = f[q];

@@ -69,9 +69,15 @@ struct md_build_ctx {
   int entry_capacity, segment_capacity;
   float list_cutoff2;
   float half_min_width;
+  float width[3];  // perpendicular widths along a, b, c
+  float fmax[3];   // bound on |fractional coordinate| of a reduced difference
   float list_cutoff;
   md_box box;
 };
+
+// image shells enumerated per axis at most (9 x 9 x 9 images); only boxes
+// not much wider than the cutoff with very sparse clusters come near it
+#define MD_MAX_IMAGE_SHELLS 4
 
 struct md_list_ptrs {
   const int64_t *cluster_center;
@@ -340,9 +346,10 @@ __device__ __forceinline__ bool md_boxes_within(float dx, float dy, float dz, fl
 
 // The candidate periodic images of cluster J relative to cluster c: the
 // minimum image of the centre difference, and -- when the box is narrow
-// compared with the list radius plus the cluster sizes -- its 26 neighbours.
-// Emits, in a fixed order, the image offsets (as lattice multiples added to
-// the reduced difference) that pass the box test.  For a cluster of the
+// compared with the list radius plus the cluster sizes -- the images within
+// K[0] x K[1] x K[2] lattice steps of it.  Visits, in a fixed order, the image
+// offsets (as lattice multiples added to the reduced difference) with a code
+// that is 0 for the reduced image and negates with the offset.  For a cluster of the
 // super-cluster itself (`own`) the images are centred on the unshifted
 // cluster instead, so that code 0 is the zero shift and the codes of a
 // shift and of its negative have opposite signs: a super-cluster that spans
@@ -350,20 +357,22 @@ __device__ __forceinline__ bool md_boxes_within(float dx, float dy, float dz, fl
 // the positive codes would then keep both of some +/- pairs.
 template <typename F>
 __device__ void md_for_each_image(const md_build_ctx &c, int64_t dx, int64_t dy, int64_t dz, bool enumerate,
-                                  bool own, F &&visit) {
+                                  bool own, const int *K, F &&visit) {
   if (!own) md_min_image_fixed(dx, dy, dz, c.box);
   if (!enumerate) {
     visit(dx, dy, dz, 0);
     return;
   }
+  const int na = 2 * K[0] + 1, nb = 2 * K[1] + 1;
+  const int centre = (K[2] * nb + K[1]) * na + K[0];
   int code = 0;
-  for (int kc = -1; kc <= 1; ++kc)
-    for (int kb = -1; kb <= 1; ++kb)
-      for (int ka = -1; ka <= 1; ++ka, ++code) {
+  for (int kc = -K[2]; kc <= K[2]; ++kc)
+    for (int kb = -K[1]; kb <= K[1]; ++kb)
+      for (int ka = -K[0]; ka <= K[0]; ++ka, ++code) {
         int64_t ex = dx + ka * c.box.fa[0] + kb * c.box.fb[0] + kc * c.box.fc[0];
         int64_t ey = dy + kb * c.box.fb[1] + kc * c.box.fc[1];
         int64_t ez = dz + kc * c.box.fc[2];
-        visit(ex, ey, ez, code - 13);  // 0 = the reduced image itself
+        visit(ex, ey, ez, code - centre);  // 0 = the reduced image itself
       }
 }
 
@@ -371,7 +380,7 @@ __device__ void md_for_each_image(const md_build_ctx &c, int64_t dx, int64_t dy,
 // periodic image, `emit(imask, self, shift)` in a fixed order.
 template <typename F>
 __device__ void md_candidate(const md_build_ctx &c, const md_list_ptrs &L, int first, int nlocal, int J,
-                             bool enumerate, const int64_t *cx, const int64_t *cy, const int64_t *cz,
+                             bool enumerate, const int *K, const int64_t *cx, const int64_t *cy, const int64_t *cz,
                              const float4 *ch, F &&emit) {
   int64_t jx = L.cluster_center[J], jy = L.cluster_center[c.nclusters + J], jz = L.cluster_center[2 * c.nclusters + J];
   float4 jh = L.cluster_half[J];
@@ -379,7 +388,7 @@ __device__ void md_candidate(const md_build_ctx &c, const md_list_ptrs &L, int f
   // S's own clusters keep their unshifted frame (see md_for_each_image)
   int64_t dx = jx - cx[0], dy = jy - cy[0], dz = jz - cz[0];
   bool own = J < first + 8;
-  md_for_each_image(c, dx, dy, dz, enumerate, own, [&](int64_t ex, int64_t ey, int64_t ez, int code) {
+  md_for_each_image(c, dx, dy, dz, enumerate, own, K, [&](int64_t ex, int64_t ey, int64_t ez, int code) {
     // the lattice vector that moves J into this image
     int64_t shx = ex - dx, shy = ey - dy, shz = ez - dz;
     bool zero = shx == 0 && shy == 0 && shz == 0;
@@ -415,10 +424,26 @@ __global__ void __launch_bounds__(64) k_nb_list(const md_nb_args *__restrict__ T
   int lane = threadIdx.x;
   if (S >= c.nsuper) return;
   float reach = __uint_as_float(ctl->cluster_reach_bits[0]) + __uint_as_float(ctl->cluster_reach_bits[1]);
-  bool enumerate = c.list_cutoff + reach > c.half_min_width;
-  // the 27 images around the minimum image hold every image within the
-  // narrowest width of the first cluster; beyond that, refuse
-  if (PASS == 0 && S == 0 && lane == 0 && c.list_cutoff + reach > 2.0f * c.half_min_width) ctl->overflow = 2;
+  // Every image of a j-cluster that can come within the list radius of S lies
+  // within R of S's first cluster.  Below half the narrowest width that is the
+  // minimum image alone.  Otherwise an image K_i + 1 or more lattice steps
+  // along axis i has fractional coordinate at least K_i + 1 - f_i there (f_i
+  // bounds the difference it was stepped from: `fmax` for a reduced one,
+  // reach/w_i for S's own clusters, which are not reduced), so it lies at least
+  // (K_i + 1 - f_i) w_i away: K_i = ceil(R / w_i + f_i - 1) shells suffice.
+  // (An own cluster's centre is within `reach` of the first cluster's.)
+  const float R = c.list_cutoff + reach;
+  const bool enumerate = R > c.half_min_width;
+  int K[3] = {1, 1, 1};
+  bool too_far = false;
+  if (enumerate)
+    for (int d = 0; d < 3; ++d) {
+      float f = fmaxf(c.fmax[d], reach / c.width[d]);
+      int k = max(1, (int)ceilf(R / c.width[d] + f - 1.0f));
+      too_far |= k > MD_MAX_IMAGE_SHELLS;
+      K[d] = min(k, MD_MAX_IMAGE_SHELLS);
+    }
+  if (PASS == 0 && S == 0 && lane == 0 && too_far) ctl->overflow = 2;
   int first = S * 8;
   int nlocal = min(8, c.nclusters - first);
   int64_t cx[8], cy[8], cz[8];
@@ -438,7 +463,7 @@ __global__ void __launch_bounds__(64) k_nb_list(const md_nb_args *__restrict__ T
     bool valid = J < c.nclusters;
     int count = 0;
     if (valid)
-      md_candidate(c, L, first, nlocal, J, enumerate, cx, cy, cz, ch,
+      md_candidate(c, L, first, nlocal, J, enumerate, K, cx, cy, cz, ch,
                    [&](int, int, int64_t, int64_t, int64_t) { ++count; });
     int incl = count;
     for (int off = 1; off < 64; off <<= 1) {
@@ -449,7 +474,7 @@ __global__ void __launch_bounds__(64) k_nb_list(const md_nb_args *__restrict__ T
       int at = base_out + written + incl - count;
       int SJ = J / 8;
       int64_t jox = L.super_origin[SJ], joy = L.super_origin[c.nsuper + SJ], joz = L.super_origin[2 * c.nsuper + SJ];
-      md_candidate(c, L, first, nlocal, J, enumerate, cx, cy, cz, ch,
+      md_candidate(c, L, first, nlocal, J, enumerate, K, cx, cy, cz, ch,
                    [&](int imask, int self, int64_t shx, int64_t shy, int64_t shz) {
                      if (at < c.entry_capacity) {
                        L.entry_j[at] = J;
@@ -718,10 +743,15 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
   // left, across every replica's list.  Each segment's energy is rounded to
   // Q32.32 and added on its own, so the sums do not depend on which wave took
   // which segment.
+  //
+  // The segment number is taken with every lane in the atomic (lane 0 adds 1,
+  // the others 0) and made wave-uniform with readfirstlane, so the loop's exit
+  // is a scalar branch.  A lane-0-only atomic followed by a __shfl broadcast is
+  // not safe here: the optimiser threads the per-segment `lane == 0` energy
+  // flush below into the next iteration's `lane == 0` atomic, after which
+  // lanes 1-63 loop back to the broadcast without lane 0 and never leave.
   for (;;) {
-    int g = 0;
-    if (lane == 0) g = atomicAdd(cursor, 1);
-    g = __shfl(g, 0, 64);
+    const int g = __builtin_amdgcn_readfirstlane(atomicAdd(cursor, lane == 0 ? 1 : 0));
     if (g >= nseg) break;
     int seg;
     const md_pair_args &A = T[md_locate(seg_map, g, seg)].P;
@@ -907,11 +937,11 @@ void md_nb_setup(md_system *s) {
   double ext = std::cbrt(MD_CLUSTER / std::max(density, 1.0));
   // a super-cluster (a compact run of the cell walk, side ~ (64/density)^1/3)
   // meets the j-clusters within the list radius plus a cluster's extent of its
-  // box: 1.5x that count, or every cluster at most (27 images of each)
+  // box: 1.5x that count, or every cluster at most (729 images of each)
   double side = std::cbrt(MD_SUPER / std::max(density, 1.0));
   double span = side + 2.0 * (rlist + ext);
   double per_super = span * span * span * density / MD_CLUSTER;
-  long cap = (long)(1.5 * B.nsuper * std::min(per_super, 27.0 * nb.nclusters)) + 1024;
+  long cap = (long)(1.5 * B.nsuper * std::min(per_super, 729.0 * nb.nclusters)) + 1024;
   B.entry_capacity = (int)std::min(cap, (long)INT32_MAX / 16);
   B.segment_capacity = B.entry_capacity / MD_SEGMENT + B.nsuper + 64;
   B.entry_j = md_alloc<int>(s, B.entry_capacity);
@@ -957,8 +987,19 @@ static md_build_ctx build_ctx(md_system *s) {
     return std::sqrt(x * x + y * y + z * z);
   };
   double v = b[0] * b[4] * b[8];
-  c.half_min_width =
-      (float)(0.5 * std::min(v / cross_norm(b + 3, b + 6), std::min(v / cross_norm(b + 6, b), v / cross_norm(b, b + 3))));
+  const double wa = v / cross_norm(b + 3, b + 6), wb = v / cross_norm(b + 6, b), wc = v / cross_norm(b, b + 3);
+  c.half_min_width = (float)(0.5 * std::min(wa, std::min(wb, wc)));
+  // md_min_image_fixed rounds along c, then b, then a on Cartesian components,
+  // which leaves the fractional coordinates of the reduced difference within
+  // fc = 1/2, fb = 1/2 + fc |cy|/by and fa = 1/2 + (fb |bx| + fc |cx|)/ax (plus
+  // a margin for the f32 rounding); a rectangular box gives 1/2 on every axis
+  const double fc = 0.5, fb = 0.5 + fc * std::fabs(b[7]) / b[4],
+               fa = 0.5 + (fb * std::fabs(b[3]) + fc * std::fabs(b[6])) / b[0];
+  const double w[3] = {wa, wb, wc}, f[3] = {fa, fb, fc};
+  for (int d = 0; d < 3; ++d) {
+    c.width[d] = (float)(w[d] * (1.0 - 1e-5));
+    c.fmax[d] = (float)(f[d] + 1e-5);
+  }
   c.box = s->dbox;
   return c;
 }

@@ -144,6 +144,18 @@ def library():
         L.md_batch_destroy.restype = None
         L.md_batch_run.argtypes = [C.c_void_p, C.c_int64]
         L.md_batch_profile_steps.argtypes = [C.c_void_p, C.c_int32, _dp]
+        _fp = P(C.c_float)
+        L.md_warmstart_create.argtypes = [C.c_int32, C.c_uint32, C.c_int64, P(C.c_void_p)]
+        L.md_warmstart_destroy.argtypes = [C.c_void_p]
+        L.md_warmstart_destroy.restype = None
+        L.md_warmstart_param_count.restype = C.c_int32
+        L.md_warmstart_record.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+        L.md_warmstart_sample_count.argtypes = [C.c_void_p, P(C.c_int64)]
+        L.md_warmstart_train.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_double, _dp]
+        L.md_warmstart_attach.argtypes = [C.c_void_p, C.c_void_p]
+        L.md_warmstart_params.argtypes = [C.c_void_p, _fp, _fp]
+        L.md_warmstart_evaluate.argtypes = [C.c_void_p, _fp, _fp, C.c_int32, _fp, _fp, _dp]
+        L.md_shake_counts.argtypes = [C.c_void_p, P(C.c_int64), P(C.c_int64)]
         _lib = L
     return _lib
 
@@ -406,3 +418,80 @@ def probe_thermostat(seed: int, step: int, n: int):
     check(library().md_probe_thermostat(seed, step, n, bits.ctypes.data_as(C.POINTER(C.c_uint32)),
                                         normals.ctypes.data_as(C.POINTER(C.c_float))))
     return bits.reshape(n, 4), normals.reshape(n, 3)
+
+
+def _f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float)) if a is not None else None
+
+
+class NativeWarmStart:
+    """An ``md_warmstart*``: the learned SHAKE warm-start network, its
+    recorded training samples and its optimiser state, on one device."""
+
+    F, MAXC, O = 32, 4, 16
+
+    def __init__(self, device=0, seed=0, capacity=1_000_000):
+        h = C.c_void_p()
+        check(library().md_warmstart_create(int(device), int(seed) & 0xFFFFFFFF, int(capacity), C.byref(h)))
+        self._h = h
+        self.nparams = int(library().md_warmstart_param_count())
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            library().md_warmstart_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def record(self, system: "NativeSystem", steps: int):
+        check(library().md_warmstart_record(self._h, system.handle, int(steps)))
+
+    @property
+    def samples(self) -> int:
+        v = C.c_int64()
+        check(library().md_warmstart_sample_count(self._h, C.byref(v)))
+        return v.value
+
+    def train(self, iterations: int, batch: int = 4096, learning_rate: float = 1e-3) -> float:
+        loss = C.c_double()
+        check(library().md_warmstart_train(self._h, int(iterations), int(batch), float(learning_rate),
+                                           C.byref(loss)))
+        return loss.value
+
+    def attach(self, system: "NativeSystem"):
+        check(library().md_warmstart_attach(system.handle, self._h))
+
+    @staticmethod
+    def detach(system: "NativeSystem"):
+        check(library().md_warmstart_attach(system.handle, None))
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.nparams, dtype=np.float32)
+        check(library().md_warmstart_params(self._h, _f(out), None))
+        return out
+
+    def set_params(self, theta):
+        t = np.ascontiguousarray(theta, dtype=np.float32)
+        check(library().md_warmstart_params(self._h, None, _f(t)))
+
+    def evaluate(self, x, t, outputs=True, gradient=True):
+        """(loss, outputs [n, 16] or None, gradient [nparams] or None)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        n = x.shape[0]
+        y = np.zeros((n, self.O), dtype=np.float32) if outputs else None
+        g = np.zeros(self.nparams, dtype=np.float32) if gradient else None
+        loss = C.c_double()
+        check(library().md_warmstart_evaluate(self._h, _f(x), _f(t), n, _f(y), _f(g), C.byref(loss)))
+        return loss.value, y, g
+
+
+def shake_counts(system: "NativeSystem"):
+    """(SHAKE update passes, cluster-steps) of a system so far."""
+    p, c = C.c_int64(), C.c_int64()
+    check(library().md_shake_counts(system.handle, C.byref(p), C.byref(c)))
+    return p.value, c.value

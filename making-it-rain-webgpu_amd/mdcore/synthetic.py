@@ -75,6 +75,14 @@ class _Topology:
     def n(self):
         return len(self.pos)
 
+    def truncate(self, n: int, n_cons: int, key):
+        """Drop the atoms from index n on and the constraints from n_cons on."""
+        for i in range(n, len(self.pos)):
+            self.grid[key(self.pos[i])].remove(i)
+        for lst in (self.pos, self.mass, self.q, self.sig, self.eps):
+            del lst[n:]
+        del self.cons[n_cons:]
+
 
 def _angle(a, b, c):
     u, v = a - b, c - b
@@ -97,7 +105,10 @@ def _chain(top: _Topology, rng, n_atoms: int, center, radius: float, wall: float
     angled to where the walk can manage it (else the roomiest of 600 tries);
     an atom's hydrogens are placed once both its heavy neighbours are known,
     in the sp3 slots those bonds leave (two on an inner atom, three on an
-    end), so every H-C-X angle starts near 109.5 degrees."""
+    end), so every H-C-X angle starts near 109.5 degrees.  When no try
+    leaves 0.22 nm around the new atom and 0.17 nm around the hydrogen slots
+    it fixes, the walk steps back (one atom, then more if the same step keeps
+    failing) and grows again from there."""
     start = top.n
     heavy: List[int] = []
     hs_of = {}
@@ -188,7 +199,16 @@ def _chain(top: _Topology, rng, n_atoms: int, center, radius: float, wall: float
             top.cons.append((i, hs[-1], 0.109))
         hs_of[i] = hs
 
-    for k in range(len(counts)):
+    marks = []  # (atoms, constraints, walk direction) before each step
+    fails = {}
+    backtracks = 0
+    k = 0
+    while k < len(counts):
+        mark = (top.n, len(top.cons), prev_dir.copy())
+        if k < len(marks):
+            marks[k] = mark
+        else:
+            marks.append(mark)
         if k == 0:
             # the first atom: near the centre, clear of anything already built
             p, room = None, -1.0
@@ -230,6 +250,19 @@ def _chain(top: _Topology, rng, n_atoms: int, center, radius: float, wall: float
                     p, best, room = q, d, r
                 if r >= 0.25:
                     break
+            if room < 0.22 and k >= 2 and backtracks < 4 * len(counts):
+                # no room here: undo the last `depth` steps and grow again
+                backtracks += 1
+                fails[k] = fails.get(k, 0) + 1
+                depth = min(k - 1, 1 + fails[k] // 8)
+                n0, c0, d0 = marks[k - depth]
+                top.truncate(n0, c0, key)
+                for j in heavy[k - depth - 1:k - 1]:
+                    hs_of.pop(j, None)  # their hydrogens went in with the undone steps
+                del heavy[k - depth:]
+                prev_dir = d0
+                k -= depth
+                continue
             if best is None:  # every try crossed the wall: turn inwards
                 best = inmost
                 p = top.pos[last] + 0.153 * best
@@ -239,6 +272,7 @@ def _chain(top: _Topology, rng, n_atoms: int, center, radius: float, wall: float
         # the previous atom now has both neighbours: its hydrogens go in
         if k >= 1:
             place_hydrogens(heavy[-2], counts[k - 1], heavy[max(0, k - 2):k - 1] + [i])
+        k += 1
     place_hydrogens(heavy[-1], counts[-1], heavy[-2:-1])
     # bonded terms from the built geometry
     for a, b in zip(heavy[:-1], heavy[1:]):

@@ -147,28 +147,34 @@ def test_boundary_frame_counts_each_pair_once(c0):
     s.close()
 
 
-def test_forces_are_invariant_under_translations(c0):
+def test_forces_follow_the_oracle_under_translations(c0):
     """Whole-system translations move every cluster across the periodic
-    boundaries in turn; the forces (and the oracle's) do not change.  On an
-    equilibrated frame: a raw generated geometry has contacts whose r^-13
-    forces turn f32 position rounding into 1e-5 relative force changes."""
+    boundaries in turn.  At every translated frame the forces match the
+    oracle's at that same frame -- the mesh term is not translation
+    invariant (the oracle's own forces move by ~5e-5 relative between these
+    frames) -- and the direct-space energy, which is invariant, stays within
+    the energy gate of the untranslated frame's.  On an equilibrated frame:
+    a raw generated geometry has contacts whose r^-13 forces turn f32
+    position rounding into 1e-5 relative force changes."""
     x = np.load(os.path.join(GOLDEN, "boundary-frame-config0.npy"))
     o = O.OracleSystem(c0.system)
     grid = o.default_grid(c0.system.boxVectors)
-    _, og, of = o.potential(x, c0.system.boxVectors, grid=grid)
+    _, og, _ = o.potential(x, c0.system.boxVectors, grid=grid)
     s = native.NativeSystem(c0.system, mesh=grid)
     rng = np.random.default_rng(7)
     L = np.asarray(c0.system.boxVectors)
     worst = 0.0
-    for k in range(24):
+    for k in range(12):
         t = rng.uniform(-1.0, 1.0, 3) @ L
-        s.set_positions((x.reshape(-1, 3) + t).reshape(-1))
+        xt = (x.reshape(-1, 3) + t).reshape(-1)
+        _, _, of = o.potential(xt, c0.system.boxVectors, grid=grid)
+        s.set_positions(xt)
         _, groups, forces = s.evaluate()
         err = rel_rms(forces, of)
         worst = max(worst, err)
         assert err < FORCE_RMS, (k, t, err)
         assert abs(groups[3] - og[3]) < ENERGY_REL * abs(og[3]), (k, groups[3], og[3])
-    print("worst force rel RMS over 24 translations", worst)
+    print("worst force rel RMS over 12 translations", worst)
     s.close()
 
 
