@@ -15,11 +15,17 @@ same replicas are timed two ways for K steps each:
 * plain: one after another, each through its own single-system engine
   (strides of 50 steps captured as HIP graphs);
 * batched: all 64 together, one kernel launch per phase for every replica,
-  strides captured as HIP graphs.
+  strides captured as HIP graphs, plain SHAKE;
+* batched + learned: the same, after a 32-64-64-16 network has been trained
+  on SHAKE multipliers recorded from every replica (outside the timed
+  region; its cost is reported) and attached to every replica, so each
+  constraint cluster's SHAKE starts from the network's guess.
 
-`value` is the batched aggregate ns/day: (sum over replicas of K * dt) / wall
-time.  The forces of sampled replicas are checked against the f64 oracle
-after the batched run (the metric's fixed force error, 2e-6 relative RMS).
+`value` is the batched + learned aggregate ns/day: (sum over replicas of K *
+dt) / wall time.  Constraints converge to the same tolerance either way; the
+forces of sampled replicas are checked against the f64 oracle after the run
+(the metric's fixed force error, 2e-6 relative RMS) and every constrained
+distance against its length.
 
 Workload "single": the 26,749-atom system alone (truncated octahedron), the
 plain path's single-system throughput.
@@ -102,6 +108,26 @@ def force_error(engine, system):
     return float(np.sqrt(np.sum((f - of) ** 2) / np.sum(of ** 2)))
 
 
+def shake_rate(engines, before):
+    """SHAKE passes per cluster-step over `engines` since the counts `before`."""
+    from mdcore import native
+    passes = clusters = 0
+    for e, (p0, c0) in zip(engines, before):
+        p1, c1 = native.shake_counts(e._native)
+        passes += p1 - p0
+        clusters += c1 - c0
+    return passes / clusters if clusters else 0.0
+
+
+def constraint_error(engine, system):
+    """Largest relative deviation of a constrained distance from its length."""
+    if system.constraintCount == 0:
+        return 0.0
+    x = engine.positions().reshape(-1, 3)
+    d = np.linalg.norm(x[system.constraintP1] - x[system.constraintP2], axis=1)
+    return float(np.max(np.abs(d - system.constraintDistance) / system.constraintDistance))
+
+
 def pair_roofline(pairs, kernel_ms):
     achieved = pairs * PAIR_FLOPS / (kernel_ms * 1e-3) / 1e12 if kernel_ms > 0 else 0.0
     return {
@@ -154,18 +180,56 @@ def run_ensemble(args):
     plain_nsday = len(engines) * plain_steps * DT_PS * 1e-3 / plain_s * 86400.0
     log(f"plain: {len(engines)} x {plain_steps} steps in {plain_s:.2f} s = {plain_nsday:.1f} ns/day aggregate")
 
-    # ---- batched ---------------------------------------------------------------
+    # ---- batched, plain SHAKE ----------------------------------------------------
+    from mdcore import native
     batch = HipBatch(engines)
     batch.run(max(args.warmup, 50))
     device_synchronize()
+    counts0 = [native.shake_counts(e._native) for e in engines]
+    t0 = time.perf_counter()
+    batch.run(args.steps)
+    device_synchronize()
+    cold_s = time.perf_counter() - t0
+    cold_rate = shake_rate(engines, counts0)
+    cold_nsday = len(engines) * args.steps * DT_PS * 1e-3 / cold_s * 86400.0
+    log(f"batched, plain SHAKE: {args.steps} steps of all replicas in {cold_s:.2f} s = "
+        f"{cold_nsday:.1f} ns/day aggregate, {cold_nsday / plain_nsday:.2f}x plain, "
+        f"{cold_rate:.3f} SHAKE passes per cluster-step")
+
+    # ---- the learned SHAKE warm start: record, train, attach ---------------------
+    t = time.perf_counter()
+    ws = native.NativeWarmStart(seed=args.seed, capacity=args.ws_capacity)
+    for e in engines:
+        ws.record(e._native, args.record_steps)
+    loss0 = ws.train(1, args.train_batch, args.learning_rate)
+    loss1 = ws.train(args.train_iterations, args.train_batch, args.learning_rate)
+    for e in engines:
+        ws.attach(e._native)
+    device_synchronize()
+    train_s = time.perf_counter() - t
+    log(f"warm start: {ws.samples} samples from {args.record_steps} steps of every replica, "
+        f"{args.train_iterations} Adam steps (loss {loss0:.4g} -> {loss1:.4g}) in {train_s:.1f} s")
+
+    # ---- batched + learned: the measured configuration ---------------------------
+    batch.run(max(args.warmup, 50))
+    device_synchronize()
+    counts0 = [native.shake_counts(e._native) for e in engines]
     t0 = time.perf_counter()
     batch.run(args.steps)
     device_synchronize()
     batch_s = time.perf_counter() - t0
+    warm_rate = shake_rate(engines, counts0)
     batch_nsday = len(engines) * args.steps * DT_PS * 1e-3 / batch_s * 86400.0
     ms_per_step = batch_s / args.steps * 1e3
-    log(f"batched: {args.steps} steps of all replicas in {batch_s:.2f} s = {ms_per_step:.3f} ms/step, "
-        f"{batch_nsday:.1f} ns/day aggregate, {batch_nsday / plain_nsday:.2f}x plain")
+    log(f"batched + learned: {args.steps} steps of all replicas in {batch_s:.2f} s = {ms_per_step:.3f} ms/step, "
+        f"{batch_nsday:.1f} ns/day aggregate, {batch_nsday / plain_nsday:.2f}x plain, "
+        f"{warm_rate:.3f} SHAKE passes per cluster-step")
+    deviations = [constraint_error(e, r.built.system) for e, r in zip(engines, reps)]
+    worst_constraint = max(deviations)
+    log(f"worst relative constraint deviation after the learned run: {worst_constraint:.2e}")
+    if worst_constraint > 1e-4:
+        bad = [(atoms[k], f"{d:.1e}") for k, d in enumerate(deviations) if d > 1e-4]
+        log(f"WARNING: replicas whose constraints no longer hold (atoms, deviation): {bad}")
 
     # ---- fixed force error: sampled replicas against the oracle ---------------
     order = np.argsort(atoms)
@@ -196,17 +260,26 @@ def run_ensemble(args):
         "dtype": "fp32",
         "data": "synthetic (seeded generator: chains + TIP3P + ions), minimised and thermalised",
         "config": {
-            "workload": f"batched ensemble: {len(engines)} replicas, {total_atoms} atoms (log-uniform "
-                        f"{min(atoms)}-{max(atoms)}), cubic/octahedral cells, 290-320 K, PME 1.0 nm, LFMiddle, "
-                        "dt 2 fs, SHAKE + SETTLE; one launch per phase for all replicas, 50-step HIP-graph strides",
+            "workload": f"batched ensemble + learned SHAKE warm start: {len(engines)} replicas, {total_atoms} atoms "
+                        f"(log-uniform {min(atoms)}-{max(atoms)}), cubic/octahedral cells, 290-320 K, PME 1.0 nm, "
+                        "LFMiddle, dt 2 fs, SHAKE + SETTLE; one launch per phase for all replicas, 50-step "
+                        "HIP-graph strides",
             "replicas": len(engines),
             "atoms": total_atoms,
             "plain_ns_day": plain_nsday,
             "plain_seconds": plain_s,
             "plain_steps_per_replica": plain_steps,
-            "batched_over_plain": batch_nsday / plain_nsday,
+            "batched_plain_shake_ns_day": cold_nsday,
+            "batched_learned_over_plain": batch_nsday / plain_nsday,
+            "batched_over_plain": cold_nsday / plain_nsday,
+            "learned_over_batched": batch_nsday / cold_nsday,
             "atom_steps_per_s": total_atoms * args.steps / batch_s,
-            "learned_warm_start": "not in this measurement",
+            "shake_passes_per_cluster_step": {"plain": cold_rate, "learned": warm_rate},
+            "learned_warm_start": {"samples": int(ws.samples), "record_steps_per_replica": args.record_steps,
+                                   "adam_steps": args.train_iterations, "batch": args.train_batch,
+                                   "loss_first": loss0, "loss_last": loss1, "seconds": train_s,
+                                   "network": "32-64-64-16 tanh MLP, bf16 MFMA 16x16x16, fp32 accumulation"},
+            "worst_constraint_rel_deviation": worst_constraint,
             "force_rel_rms_vs_oracle": errors,
             "in_cutoff_pairs": int(pairs),
             "arithmetic": "fp32 with Q32.32 int64 accumulation",
@@ -233,7 +306,9 @@ def run_ensemble(args):
         log(f"cpu baseline {cpu_nsday:.5f} ns/day aggregate")
     batch.destroy()
     for e in engines:
+        native.NativeWarmStart.detach(e._native)
         e.destroy()
+    ws.close()
     return result
 
 
@@ -295,6 +370,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--minimise-iterations", type=int, default=60)
+    ap.add_argument("--record-steps", type=int, default=25, help="warm-start samples: steps recorded per replica")
+    ap.add_argument("--train-iterations", type=int, default=1500)
+    ap.add_argument("--train-batch", type=int, default=4096)
+    ap.add_argument("--learning-rate", type=float, default=1e-3)
+    ap.add_argument("--ws-capacity", type=int, default=4_000_000)
+    ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--max-batch", type=int, default=0,
                     help="steps per captured HIP graph (0: 50; 1: launch every step directly)")
     args = ap.parse_args()

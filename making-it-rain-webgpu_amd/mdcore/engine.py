@@ -366,15 +366,98 @@ def _max_constraint_error(system, x):
     return float(np.max(np.abs(r - d) / d))
 
 
+def lbfgs(objective, x, max_iterations: int = 0, rms_tolerance: float = 10.0, particles: int = 1,
+          memory: int = 6, max_move: float = 0.5, max_line_search: int = 40, armijo: float = 1e-4,
+          curvature: float = 0.9):
+    """L-BFGS with a backtracking strong-Wolfe line search, the search OpenMM
+    runs through libLBFGS and the reference follows (src/reference/lbfgs.ts
+    :102-275): `memory` correction pairs, a first step of 1/|d| and later
+    steps of 1, the step halved when sufficient decrease fails or the slope
+    overshoots and multiplied by 2.1 while the slope stays steep.  Stops when
+    the RMS gradient per particle is at most `rms_tolerance` (libLBFGS's
+    |g|/max(|x|,1) test with OpenMM's epsilon), after `max_iterations` (0: no
+    limit), or when the line search cannot make progress, keeping the last
+    accepted point.  One addition: a trial point that moves any coordinate
+    more than `max_move` nm from the last accepted point counts as a failed
+    decrease and is not evaluated, and a step with sufficient decrease that
+    could only grow past that limit is taken as it is, so the search cannot
+    throw atoms across the cell from a clashed start.
+
+    `objective(x) -> (value, gradient)`; returns (x, value, iterations)."""
+    x = np.array(x, dtype=np.float64, copy=True)
+    value, g = objective(x)
+
+    def done(grad):
+        return math.sqrt(float(grad @ grad) / particles) <= rms_tolerance
+
+    if done(g):
+        return x, value, 0
+    S, Y, RHO = [], [], []
+    d = -g
+    step = 1.0 / float(np.linalg.norm(d))
+    iteration = 0
+    while True:
+        iteration += 1
+        slope0 = float(g @ d)
+        if slope0 >= 0.0:
+            return x, value, iteration - 1
+        accepted = False
+        for _ in range(max_line_search):
+            trial = x + step * d
+            if float(np.max(np.abs(step * d))) > max_move:
+                step *= 0.5
+                continue
+            v, gt = objective(trial)
+            if not v <= value + armijo * step * slope0:
+                step *= 0.5
+                continue
+            slope = float(gt @ d)
+            if slope < curvature * slope0:
+                if 2.1 * step * float(np.max(np.abs(d))) > max_move:
+                    accepted = True  # still steep, but at the displacement cap
+                    break
+                step *= 2.1
+            elif slope > -curvature * slope0:
+                step *= 0.5
+            else:
+                accepted = True
+                break
+        if not accepted:
+            return x, value, iteration - 1
+        s, y = trial - x, gt - g
+        x, value, g = trial, v, gt
+        if done(g) or (max_iterations and iteration >= max_iterations):
+            return x, value, iteration
+        sy = float(s @ y)
+        if sy > 0.0:
+            S.append(s)
+            Y.append(y)
+            RHO.append(1.0 / sy)
+            if len(S) > memory:
+                S.pop(0), Y.pop(0), RHO.pop(0)
+        # two-loop recursion: d = -H g
+        q = -g.copy()
+        alphas = []
+        for s_k, y_k, r_k in zip(reversed(S), reversed(Y), reversed(RHO)):
+            a = r_k * float(s_k @ q)
+            alphas.append(a)
+            q -= a * y_k
+        if S:
+            q *= 1.0 / (RHO[-1] * float(Y[-1] @ Y[-1]))
+        for (s_k, y_k, r_k), a in zip(zip(S, Y, RHO), reversed(alphas)):
+            b = r_k * float(y_k @ q)
+            q += (a - b) * s_k
+        d = q
+        step = 1.0
+
+
 def minimise(engine: HipEngine, maxIterations: int = 0, tolerance: float = 10.0) -> MinimiseResult:
     """OpenMM's LocalEnergyMinimizer objective: U(x) + sum (k/2)(r - d)^2 over
     the constraints, k = 100 / max(1e-4, tol), stiffened tenfold per round
     until every constraint is within max(1e-4, tol); gives up to the starting
     positions when stiffening stops helping; then projects to the caller's
-    tolerance (src/reference/minimise.ts:164-280).  The search is L-BFGS with
-    memory 6 (libLBFGS's default, which OpenMM keeps)."""
-    from scipy.optimize import minimize
-
+    tolerance (src/reference/minimise.ts:164-280).  The search is `lbfgs`
+    above (memory 6, libLBFGS's default, which OpenMM keeps)."""
     system = engine.system
     n = system.particleCount
     ctol = engine.options.constraintTolerance
@@ -385,7 +468,6 @@ def minimise(engine: HipEngine, maxIterations: int = 0, tolerance: float = 10.0)
     initial = x0.copy()
     frozen = ~engine._moving
     p1, p2, dist = system.constraintP1, system.constraintP2, system.constraintDistance
-    last = {"energy": 0.0, "x": None, "rms": float("inf")}
 
     def objective(flat):
         x = flat.reshape(-1, 3)
@@ -402,16 +484,7 @@ def minimise(engine: HipEngine, maxIterations: int = 0, tolerance: float = 10.0)
             gd = (k * dr / np.where(r == 0, 1.0, r))[:, None] * d
             np.add.at(g, p1, gd)
             np.add.at(g, p2, -gd)
-        last["energy"] = energy
-        last["x"] = flat.copy()
-        # libLBFGS's |g| / max(|x|, 1) < tol / rms(|x|) is an RMS force per
-        # particle below `tolerance`
-        last["rms"] = math.sqrt(float(np.sum(g * g)) / n)
         return energy + spring, g.reshape(-1)
-
-    def converged(xk):
-        if last["x"] is not None and np.array_equal(last["x"], xk) and last["rms"] < tolerance:
-            raise StopIteration
 
     prev1 = prev2 = 1e10
     rounds = iterations = 0
@@ -419,11 +492,8 @@ def minimise(engine: HipEngine, maxIterations: int = 0, tolerance: float = 10.0)
     x = x0.reshape(-1).copy()
     while True:
         rounds += 1
-        res = minimize(objective, x, jac=True, method="L-BFGS-B", callback=converged,
-                       options={"maxcor": 6, "maxiter": maxIterations if maxIterations > 0 else 100000,
-                                "gtol": 0.0, "ftol": 0.0})
-        x = res.x
-        iterations += int(res.nit)
+        x, _, nit = lbfgs(objective, x, maxIterations, tolerance, n)
+        iterations += nit
         err = _max_constraint_error(system, x.reshape(-1, 3))
         if err <= work or len(p1) == 0:
             break
