@@ -108,6 +108,17 @@ def force_error(engine, system):
     return float(np.sqrt(np.sum((f - of) ** 2) / np.sum(of ** 2)))
 
 
+def health(engines, reps, atoms, when):
+    """Log the replicas whose constraints no longer hold (a diverged run)."""
+    bad = [(atoms[k], f"{d:.1e}") for k, (e, r) in enumerate(zip(engines, reps))
+           for d in [constraint_error(e, r.built.system)] if not d < 1e-4]
+    if bad:
+        log(f"WARNING {when}: replicas whose constraints no longer hold (atoms, deviation): {bad}")
+    else:
+        log(f"{when}: every replica's constraints hold")
+    return not bad
+
+
 def shake_rate(engines, before):
     """SHAKE passes per cluster-step over `engines` since the counts `before`."""
     from mdcore import native
@@ -156,16 +167,20 @@ def run_ensemble(args):
 
     t = time.perf_counter()
     engines = []
+    gave_up = 0
     for r in reps:
         e = HipEngine(r.built.system, r.built.state,
                       RunOptions(timeStep=DT_PS, temperature=r.temperature, friction=1.0, constraintTolerance=1e-6,
                                  seed=r.seed, maxBatch=args.max_batch))
-        minimise(e, args.minimise_iterations, 10.0)
+        res = minimise(e, args.minimise_iterations, 10.0)
+        gave_up += int(res.gaveUp)
         e.setVelocitiesToTemperature(r.temperature, r.seed)
         e.run(args.warmup)
         engines.append(e)
     device_synchronize()
-    log(f"minimised, thermalised and warmed up {len(engines)} replicas in {time.perf_counter() - t:.1f} s")
+    log(f"minimised, thermalised and warmed up {len(engines)} replicas in {time.perf_counter() - t:.1f} s "
+        f"({gave_up} minimisations gave up)")
+    health(engines, reps, atoms, "after warm-up")
 
     # ---- plain: replicas one after another through the single-system path --
     plain_steps = args.plain_steps
@@ -179,6 +194,7 @@ def run_ensemble(args):
     plain_s = time.perf_counter() - t0
     plain_nsday = len(engines) * plain_steps * DT_PS * 1e-3 / plain_s * 86400.0
     log(f"plain: {len(engines)} x {plain_steps} steps in {plain_s:.2f} s = {plain_nsday:.1f} ns/day aggregate")
+    health(engines, reps, atoms, "after the plain run")
 
     # ---- batched, plain SHAKE ----------------------------------------------------
     from mdcore import native
@@ -195,12 +211,14 @@ def run_ensemble(args):
     log(f"batched, plain SHAKE: {args.steps} steps of all replicas in {cold_s:.2f} s = "
         f"{cold_nsday:.1f} ns/day aggregate, {cold_nsday / plain_nsday:.2f}x plain, "
         f"{cold_rate:.3f} SHAKE passes per cluster-step")
+    health(engines, reps, atoms, "after the batched run")
 
     # ---- the learned SHAKE warm start: record, train, attach ---------------------
     t = time.perf_counter()
     ws = native.NativeWarmStart(seed=args.seed, capacity=args.ws_capacity)
     for e in engines:
         ws.record(e._native, args.record_steps)
+    health(engines, reps, atoms, "after recording")
     loss0 = ws.train(1, args.train_batch, args.learning_rate)
     loss1 = ws.train(args.train_iterations, args.train_batch, args.learning_rate)
     for e in engines:

@@ -253,8 +253,10 @@ enum {
 md_err md_profile_steps(md_system *sys, int32_t steps, double *ms);
 const char *md_phase_name(int32_t phase);
 
-/* Work counts the benchmark uses: in-cutoff pair count (exact, counted on the
- * device at the current positions) and cluster tiles computed. */
+/* Work counts the benchmark uses, counted on the device at the current
+ * positions: non-excluded pairs inside the cutoff (exact), and the 8x8
+ * cluster-pair blocks the pair kernel evaluated (64 pair slots each), so
+ * pairs / (64 * tiles) is the fraction of evaluated pair slots that count. */
 md_err md_pair_census(md_system *sys, int64_t *pairs_within_cutoff, int64_t *tiles);
 
 /* Diagnostic probes (the upstream `random-probe` kernel, K41, and its

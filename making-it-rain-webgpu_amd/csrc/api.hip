@@ -601,6 +601,9 @@ static void check_control(md_system *s) {
     throw md_error(MD_ERR_CUTOFF_TOO_LARGE,
                    "the list radius plus the cluster extents reaches past four periodic images along an axis: "
                    "the box is too small for this cutoff");
+  if (c.overflow == 3)
+    throw md_error(MD_ERR_FIXED_POINT_OVERFLOW,
+                   "an atom moved more than 32 nm from its cluster's frame: the simulation has diverged");
   if (c.overflow)
     throw md_error(MD_ERR_LIST_OVERFLOW,
                    "the cluster-pair list outgrew its capacity (the system compressed far beyond its starting "
@@ -676,7 +679,10 @@ static void evaluate(md_system *s, double *forces, double *groups, double *total
   md_group *grp = s->self;
   group_prepare(grp);
   md_zero_forces(grp, true);
-  if (census) MD_HIP(hipMemsetAsync(&s->ctl->pair_census, 0, sizeof(int64_t), s->stream));
+  if (census) {
+    MD_HIP(hipMemsetAsync(&s->ctl->pair_census, 0, sizeof(int64_t), s->stream));
+    MD_HIP(hipMemsetAsync(&s->ctl->pair_blocks, 0, sizeof(int64_t), s->stream));
+  }
   md_step_prologue(grp, false);
   if (s->has_nonbonded) {
     md_nb_build(grp);
@@ -962,7 +968,7 @@ md_err md_pair_census(md_system *s, int64_t *pairs, int64_t *tiles) {
   md_control c;
   MD_HIP(hipMemcpy(&c, s->ctl, sizeof c, hipMemcpyDeviceToHost));
   if (pairs) *pairs = c.pair_census;
-  if (tiles) *tiles = s->has_nonbonded ? md_nb_entries(s) : 0;
+  if (tiles) *tiles = c.pair_blocks;
   MD_API_END
 }
 

@@ -47,6 +47,7 @@ struct md_control {
   uint32_t pad_;
   int64_t shake_passes;            // SHAKE update passes over all cluster-steps
   int64_t shake_clusters;          // cluster-steps SHAKE has run
+  int64_t pair_blocks;             // 8x8 cluster-pair blocks evaluated, when counted
 };
 
 struct md_nb_params {
@@ -182,7 +183,7 @@ struct md_system {
   int *slot_atom = nullptr, *atom_slot = nullptr;
   int64_t *atom_folded = nullptr, *build_pos = nullptr, *slot_offset = nullptr;
   float2 *slot_lj = nullptr;
-  float4 *slot_xyzq = nullptr;
+  int4 *slot_xyzq = nullptr;      // pair-kernel frame coordinates + charge bits
   float4 *cluster_half = nullptr;
   int *excl_start_d = nullptr, *excl_list_d = nullptr;
   void *nb_ext = nullptr;

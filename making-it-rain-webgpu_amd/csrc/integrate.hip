@@ -507,6 +507,12 @@ __device__ __forceinline__ void md_reduce_out(const md_int_args &A, uint32_t ste
   }
 }
 
+// SHAKE clusters run in workgroups of one wavefront: an ensemble has few
+// clusters per replica (~300 at 10k atoms), so 64 per workgroup spreads them
+// over four times as many CUs as 256 would, and a slow-converging cluster
+// holds up one wave, not four.
+#define MD_CLUSTER_BLOCK 64
+
 // kind 0: free atoms, 1: SETTLE waters, 2: SHAKE clusters (MAXA atoms).
 // MODE (clusters only): 0 plain SHAKE; 1 plain SHAKE, recording each
 // cluster's features and converged multipliers as training samples; 2 SHAKE
@@ -566,7 +572,7 @@ __global__ void __launch_bounds__(256) k_integrate(const md_int_args *__restrict
     bool warm = false;
     if constexpr (MODE == 2) {
       // every lane of the workgroup runs the network (its replica's, or none)
-      __shared__ float ws_lds[4 * MD_WS_TILE_LDS];
+      __shared__ float ws_lds[(MD_CLUSTER_BLOCK / 64) * MD_WS_TILE_LDS];
       if (A.ws) {
         float f[MD_WS_F], y[MD_WS_MAXC];
         if (covered)
@@ -772,7 +778,7 @@ void md_int_group_setup(md_group *g) {
   for (md_system *s : g->reps) {
     fr.push_back((s->n_free + 255) / 256);
     wa.push_back((s->n_waters + 255) / 256);
-    cl.push_back((s->n_clusters_c + 255) / 256);
+    cl.push_back((s->n_clusters_c + MD_CLUSTER_BLOCK - 1) / MD_CLUSTER_BLOCK);
     at.push_back((s->n + 255) / 256);
     g->max_cluster_atoms = std::max(g->max_cluster_atoms, s->max_cluster_atoms);
   }
@@ -822,18 +828,18 @@ void md_integrate(md_group *g) {
     const int mode = g->ws_mode;
     if (g->max_cluster_atoms <= 4) {
       if (mode == 2)
-        k_integrate<2, 4, 2><<<G.clusters.total, 256, 0, st>>>(T, G.clusters.map);
+        k_integrate<2, 4, 2><<<G.clusters.total, MD_CLUSTER_BLOCK, 0, st>>>(T, G.clusters.map);
       else if (mode == 1)
-        k_integrate<2, 4, 1><<<G.clusters.total, 256, 0, st>>>(T, G.clusters.map);
+        k_integrate<2, 4, 1><<<G.clusters.total, MD_CLUSTER_BLOCK, 0, st>>>(T, G.clusters.map);
       else
-        k_integrate<2, 4, 0><<<G.clusters.total, 256, 0, st>>>(T, G.clusters.map);
+        k_integrate<2, 4, 0><<<G.clusters.total, MD_CLUSTER_BLOCK, 0, st>>>(T, G.clusters.map);
     } else {
       if (mode == 2)
-        k_integrate<2, 8, 2><<<G.clusters.total, 256, 0, st>>>(T, G.clusters.map);
+        k_integrate<2, 8, 2><<<G.clusters.total, MD_CLUSTER_BLOCK, 0, st>>>(T, G.clusters.map);
       else if (mode == 1)
-        k_integrate<2, 8, 1><<<G.clusters.total, 256, 0, st>>>(T, G.clusters.map);
+        k_integrate<2, 8, 1><<<G.clusters.total, MD_CLUSTER_BLOCK, 0, st>>>(T, G.clusters.map);
       else
-        k_integrate<2, 8, 0><<<G.clusters.total, 256, 0, st>>>(T, G.clusters.map);
+        k_integrate<2, 8, 0><<<G.clusters.total, MD_CLUSTER_BLOCK, 0, st>>>(T, G.clusters.map);
     }
   }
   MD_HIP(hipGetLastError());
@@ -853,14 +859,14 @@ void md_constrain(md_group *g, int what, double tolerance) {
     int gsz = G.clusters.total;
     if (g->max_cluster_atoms <= 4) {
       if (what == 0)
-        k_constrain<2, 4, 0><<<gsz, 256, 0, st>>>(T, G.clusters.map, tolerance);
+        k_constrain<2, 4, 0><<<gsz, MD_CLUSTER_BLOCK, 0, st>>>(T, G.clusters.map, tolerance);
       else
-        k_constrain<2, 4, 1><<<gsz, 256, 0, st>>>(T, G.clusters.map, tolerance);
+        k_constrain<2, 4, 1><<<gsz, MD_CLUSTER_BLOCK, 0, st>>>(T, G.clusters.map, tolerance);
     } else {
       if (what == 0)
-        k_constrain<2, 8, 0><<<gsz, 256, 0, st>>>(T, G.clusters.map, tolerance);
+        k_constrain<2, 8, 0><<<gsz, MD_CLUSTER_BLOCK, 0, st>>>(T, G.clusters.map, tolerance);
       else
-        k_constrain<2, 8, 1><<<gsz, 256, 0, st>>>(T, G.clusters.map, tolerance);
+        k_constrain<2, 8, 1><<<gsz, MD_CLUSTER_BLOCK, 0, st>>>(T, G.clusters.map, tolerance);
     }
   }
   MD_HIP(hipGetLastError());

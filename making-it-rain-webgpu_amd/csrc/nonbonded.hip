@@ -87,7 +87,7 @@ struct md_list_ptrs {
   int *super_segs;    // [nsuper+1] segments per super-cluster, then starts
   int *entry_j;
   int *entry_imask;
-  float4 *entry_shift;
+  int4 *entry_shift;    // J's frame origin in S's frame, MD_FRAME_UNIT units
   int4 *segments;     // (S, begin, end, 0)
   uint64_t *entry_excl;  // [capacity][8]
 };
@@ -96,15 +96,16 @@ struct md_pair_args {
   const int4 *segments;
   const int *entry_j;
   const int *entry_imask;
-  const float4 *entry_shift;
+  const int4 *entry_shift;
   const uint64_t *entry_excl;
-  const float4 *slot_xyzq;
+  const int4 *slot_xyzq;  // frame coordinates (MD_FRAME_UNIT units) and the charge's bits
   const float2 *slot_lj;
   const int *slot_atom;
   const int *totals;
   int64_t *force;
   int64_t *energy;
-  int64_t *census;
+  int64_t *census;  // in-cutoff pairs (md_control)
+  int64_t *blocks;  // 8x8 cluster-pair blocks evaluated (md_control)
   int np, nclusters;
   float cutoff2, alpha, beta, exp_scale;
 };
@@ -120,7 +121,8 @@ struct md_nb_args {
   int *slot_atom, *atom_slot;
   int64_t *folded, *build_pos, *slot_offset, *cluster_center, *super_origin;
   float2 *slot_lj;
-  float4 *slot_xyzq, *cluster_half;
+  int4 *slot_xyzq;
+  float4 *cluster_half;
   const float4 *atom_param;
   const int *excl_start, *excl_list;
   int *totals;
@@ -130,6 +132,22 @@ struct md_nb_args {
 // this block's replica entry and its block index inside the replica
 __device__ __forceinline__ const md_nb_args &md_nb_entry(const md_nb_args *T, const md_bmap &map, int &blk) {
   return T[md_locate(map, blockIdx.x, blk)];
+}
+
+// Pair-kernel frame coordinates: int32 in units of 2^-26 nm (+/-32 nm), so a
+// pair's difference is exact integer arithmetic and is rounded once, when it
+// becomes f32 -- the reference's difference-then-convert rule (fixed64.wgsl
+// :166-216) -- instead of carrying the rounding of two offsets up to a few nm
+// long.  Overflow code 3 reports a coordinate outside the range (a diverged
+// system).
+#define MD_FRAME_SHIFT 6  // Q32.32 -> 2^-26 nm
+#define MD_FRAME_UNIT 1.4901161193847656e-08f   // 2^-26 nm
+#define MD_FRAME_UNIT2 2.220446049250313e-16f   // 2^-52 nm^2
+
+__device__ __forceinline__ int md_frame_coord(int64_t v, bool &bad) {
+  int64_t u = (v + (int64_t(1) << (MD_FRAME_SHIFT - 1))) >> MD_FRAME_SHIFT;
+  bad |= u > INT32_MAX || u < INT32_MIN;
+  return (int)u;
 }
 
 // Fold each atom into the cell and bucket it (cell-list.wgsl:139-250).  The
@@ -480,8 +498,11 @@ __global__ void __launch_bounds__(64) k_nb_list(const md_nb_args *__restrict__ T
                        L.entry_j[at] = J;
                        L.entry_imask[at] = imask | (self << 8);
                        // J's frame origin, moved into this image, in S's frame
-                       L.entry_shift[at] = make_float4(md_from_fixed(jox + shx - ox), md_from_fixed(joy + shy - oy),
-                                                       md_from_fixed(joz + shz - oz), 0.0f);
+                       bool bad = false;
+                       L.entry_shift[at] = make_int4(md_frame_coord(jox + shx - ox, bad),
+                                                     md_frame_coord(joy + shy - oy, bad),
+                                                     md_frame_coord(joz + shz - oz, bad), 0);
+                       if (bad) ctl->overflow = 3;
                      }
                      ++at;
                    });
@@ -597,32 +618,36 @@ __global__ void k_nb_pack(const md_nb_args *__restrict__ T, md_bmap map, int *se
   const int nslots = A.c.nslots, np = A.c.np;
   const int *slot_atom = A.slot_atom;
   const int64_t *pos = A.pos, *slot_offset = A.slot_offset;
-  float4 *slot_xyzq = A.slot_xyzq;
+  int4 *slot_xyzq = A.slot_xyzq;
   int s = blk * blockDim.x + threadIdx.x;
   if (s >= nslots) return;
   int a = slot_atom[s];
   if (a < 0) {
-    // far away and uncharged; every pair with it is masked anyway
-    slot_xyzq[s] = make_float4(1.0e4f + (float)(s & 63), 1.0e4f, 1.0e4f, 0.0f);
+    // far away (20 nm out) and uncharged; every pair with it is masked anyway
+    const int far = 20 << 26;
+    slot_xyzq[s] = make_int4(far + ((s & 63) << 20), far, far, 0);
     return;
   }
-  float x = md_from_fixed(pos[a] - slot_offset[s]);
-  float y = md_from_fixed(pos[np + a] - slot_offset[nslots + s]);
-  float z = md_from_fixed(pos[2 * np + a] - slot_offset[2 * nslots + s]);
-  slot_xyzq[s] = make_float4(x, y, z, A.atom_param[a].x);
+  bool bad = false;
+  int x = md_frame_coord(pos[a] - slot_offset[s], bad);
+  int y = md_frame_coord(pos[np + a] - slot_offset[nslots + s], bad);
+  int z = md_frame_coord(pos[2 * np + a] - slot_offset[2 * nslots + s], bad);
+  slot_xyzq[s] = make_int4(x, y, z, __float_as_int(A.atom_param[a].x));
+  if (bad) A.ctl->overflow = 3;
 }
 
 // ---------------------------------------------------------------------------
 // The pair kernel
 // ---------------------------------------------------------------------------
 
-// one pair from lane's i atom to the entry's j atom; returns the scalar
-// F/r factor (so F_i = -fs * d) and accumulates the pair energy
+// one pair from lane's i atom to the entry's j atom, its difference d in
+// frame units; returns the F/r factor times the frame unit (so
+// F_i = -fs * d in kJ/mol/nm) and accumulates the pair energy
 template <bool ENERGY>
 __device__ __forceinline__ float md_pair(float dx, float dy, float dz, float qi, float si, float ei, float qj,
                                          float sj, float ej, bool allowed, const md_pair_args &A, float &en,
                                          int &count) {
-  float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+  float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz)) * MD_FRAME_UNIT2;
   bool inc = allowed && r2 < A.cutoff2;
   float rinv = __builtin_amdgcn_rsqf(r2);
   float rinv2 = rinv * rinv;
@@ -638,7 +663,7 @@ __device__ __forceinline__ float md_pair(float dx, float dy, float dz, float qi,
   float s6 = s2 * s2 * s2;
   float s12 = s6 * s6;
   float flj = eps * fmaf(12.0f, s12, -6.0f * s6) * rinv2;
-  float fs = inc ? fc + flj : 0.0f;
+  float fs = inc ? (fc + flj) * MD_FRAME_UNIT : 0.0f;
   if (ENERGY) {
     float e = fmaf(qq * erfc_v, rinv, eps * (s12 - s6));
     en += inc ? e : 0.0f;
@@ -651,22 +676,23 @@ __device__ __forceinline__ float md_pair(float dx, float dy, float dz, float qi,
 // clusters.
 template <bool ENERGY>
 __device__ __forceinline__ void md_pair_segment(const md_pair_args &A, int seg, int lane, int il, int jl,
-                                                int64_t &efix, int &count) {
+                                                int64_t &efix, int &count, int &blocks) {
   int4 sg = A.segments[seg];
   int S = sg.x, eb = sg.y, ee = sg.z;
   // i data: lane's atom in each of S's 8 clusters (8 clusters x 8 atoms = 64
   // slots; lane il*8+jl reads slot S*64 + q*8 + il)
-  float xi[8], yi[8], zi[8], qi[8], si[8], ei[8];
+  int xi[8], yi[8], zi[8];
+  float qi[8], si[8], ei[8];
   float fx[8], fy[8], fz[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
     int slot = S * MD_SUPER + q * MD_CLUSTER + il;
-    float4 p = A.slot_xyzq[slot];
+    int4 p = A.slot_xyzq[slot];
     float2 lj = A.slot_lj[slot];
     xi[q] = p.x;
     yi[q] = p.y;
     zi[q] = p.z;
-    qi[q] = p.w;
+    qi[q] = __int_as_float(p.w);
     si[q] = lj.x;
     ei[q] = lj.y;
     fx[q] = fy[q] = fz[q] = 0.0f;
@@ -675,11 +701,16 @@ __device__ __forceinline__ void md_pair_segment(const md_pair_args &A, int seg, 
   for (int e = eb; e < ee; ++e) {
     int J = A.entry_j[e];
     int meta = A.entry_imask[e];
-    float4 sh = A.entry_shift[e];
+    if (ENERGY) blocks += __popc(meta & 0xff);
+    int4 sh = A.entry_shift[e];
     int jslot = J * MD_CLUSTER + jl;
-    float4 pj = A.slot_xyzq[jslot];
+    int4 pj = A.slot_xyzq[jslot];
     float2 lj = A.slot_lj[jslot];
-    float xj = pj.x + sh.x, yj = pj.y + sh.y, zj = pj.z + sh.z;
+    const float qj = __int_as_float(pj.w);
+    // wrapping (unsigned) integer arithmetic: exact for every real pair; the
+    // padding slots far out may wrap, and their pairs are masked
+    unsigned xj = (unsigned)pj.x + (unsigned)sh.x, yj = (unsigned)pj.y + (unsigned)sh.y,
+             zj = (unsigned)pj.z + (unsigned)sh.z;
     float gx = 0.0f, gy = 0.0f, gz = 0.0f;
     const uint64_t *ex = A.entry_excl + (size_t)e * 8;
 #pragma unroll
@@ -687,8 +718,9 @@ __device__ __forceinline__ void md_pair_segment(const md_pair_args &A, int seg, 
       if ((meta >> q) & 1) {
         uint64_t m = ex[q];
         bool allowed = (m >> lane) & 1;
-        float dx = xj - xi[q], dy = yj - yi[q], dz = zj - zi[q];
-        float fs = md_pair<ENERGY>(dx, dy, dz, qi[q], si[q], ei[q], pj.w, lj.x, lj.y, allowed, A, en, count);
+        float dx = (float)(int)(xj - (unsigned)xi[q]), dy = (float)(int)(yj - (unsigned)yi[q]),
+              dz = (float)(int)(zj - (unsigned)zi[q]);
+        float fs = md_pair<ENERGY>(dx, dy, dz, qi[q], si[q], ei[q], qj, lj.x, lj.y, allowed, A, en, count);
         fx[q] = fmaf(-fs, dx, fx[q]);
         fy[q] = fmaf(-fs, dy, fy[q]);
         fz[q] = fmaf(-fs, dz, fz[q]);
@@ -756,14 +788,15 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
     int seg;
     const md_pair_args &A = T[md_locate(seg_map, g, seg)].P;
     int64_t efix = 0;
-    int count = 0;
-    md_pair_segment<ENERGY>(A, seg, lane, il, jl, efix, count);
+    int count = 0, blocks = 0;
+    md_pair_segment<ENERGY>(A, seg, lane, il, jl, efix, count, blocks);
     if (ENERGY) {
       int64_t ef = md_wave_sum_i64(efix);
       int64_t cf = md_wave_sum_i64((int64_t)count);
       if (lane == 0) {
         md_atomic_add_i64(A.energy + MD_E_PAIR, ef);
         md_atomic_add_i64(A.census, cf);
+        md_atomic_add_i64(A.blocks, (int64_t)blocks);  // blocks is wave-uniform
       }
     }
   }
@@ -853,7 +886,7 @@ struct md_nb_buffers {
   int64_t *cluster_center = nullptr, *super_origin = nullptr;
   int *super_count = nullptr, *super_segs = nullptr;
   int *entry_j = nullptr, *entry_imask = nullptr;
-  float4 *entry_shift = nullptr;
+  int4 *entry_shift = nullptr;
   int4 *segments = nullptr;
   uint64_t *entry_excl = nullptr;
   int *totals = nullptr;
@@ -923,7 +956,7 @@ void md_nb_setup(md_system *s) {
   s->build_pos = md_alloc<int64_t>(s, 3 * (size_t)np);
   s->slot_offset = md_alloc<int64_t>(s, 3 * (size_t)nb.nslots);
   s->slot_lj = md_alloc<float2>(s, nb.nslots);
-  s->slot_xyzq = md_alloc<float4>(s, nb.nslots);
+  s->slot_xyzq = md_alloc<int4>(s, nb.nslots);
   s->cluster_half = md_alloc<float4>(s, nb.nclusters);
   B.cluster_center = md_alloc<int64_t>(s, 3 * (size_t)nb.nclusters);
   B.super_origin = md_alloc<int64_t>(s, 3 * (size_t)B.nsuper);
@@ -946,7 +979,7 @@ void md_nb_setup(md_system *s) {
   B.segment_capacity = B.entry_capacity / MD_SEGMENT + B.nsuper + 64;
   B.entry_j = md_alloc<int>(s, B.entry_capacity);
   B.entry_imask = md_alloc<int>(s, B.entry_capacity);
-  B.entry_shift = md_alloc<float4>(s, B.entry_capacity);
+  B.entry_shift = md_alloc<int4>(s, B.entry_capacity);
   B.entry_excl = md_alloc<uint64_t>(s, (size_t)B.entry_capacity * 8);
   B.segments = md_alloc<int4>(s, B.segment_capacity);
   s->excl_start_d = md_alloc<int>(s, s->n + 1);
@@ -1033,6 +1066,7 @@ static md_nb_args nb_args(md_system *s) {
   P.force = s->force;
   P.energy = s->energy;
   P.census = &s->ctl->pair_census;
+  P.blocks = &s->ctl->pair_blocks;
   P.np = s->np;
   P.nclusters = s->nb.nclusters;
   P.cutoff2 = s->nb.cutoff2;

@@ -101,14 +101,16 @@ def _dihedral(p0, p1, p2, p3):
 def _chain(top: _Topology, rng, n_atoms: int, center, radius: float, wall: float):
     """A self-avoiding random-walk chain of exactly n_atoms (heavy + H).
 
-    Heavy atoms are kept 0.25 nm from every atom they are not bonded or
+    Heavy atoms are kept 0.30 nm from every atom they are not bonded or
     angled to where the walk can manage it (else the roomiest of 600 tries);
     an atom's hydrogens are placed once both its heavy neighbours are known,
     in the sp3 slots those bonds leave (two on an inner atom, three on an
-    end), so every H-C-X angle starts near 109.5 degrees.  When no try
-    leaves 0.22 nm around the new atom and 0.17 nm around the hydrogen slots
-    it fixes, the walk steps back (one atom, then more if the same step keeps
-    failing) and grows again from there."""
+    end, turned to the roomiest of 12 orientations), so every H-C-X angle
+    starts near 109.5 degrees.  When no try leaves 0.27 nm around the new
+    atom and 0.21 nm around the hydrogen slots it fixes, the walk steps back
+    (one atom, then more if the same step keeps failing) and grows again from
+    there: looser limits leave hydrogens inside the 0.3 nm LJ cores of other
+    chain atoms, which minimisation turns into strained, unstable geometry."""
     start = top.n
     heavy: List[int] = []
     hs_of = {}
@@ -189,8 +191,16 @@ def _chain(top: _Topology, rng, n_atoms: int, center, radius: float, wall: float
             e1 = any_perpendicular(b)
             e2 = np.cross(b, e1)
             phi0 = rng.uniform(0, 2 * math.pi)
-            slots = [cos_t * b + sin_t * (math.cos(phi0 + t) * e1 + math.sin(phi0 + t) * e2)
-                     for t in (0.0, 2 * math.pi / 3, 4 * math.pi / 3)]
+            skip0 = {i} | set(nbrs)
+            best_slots, best_room = None, -1.0
+            for turn in range(12):
+                phi = phi0 + turn * math.pi / 18  # 10-degree turns over one third of a circle
+                cand = [cos_t * b + sin_t * (math.cos(phi + t) * e1 + math.sin(phi + t) * e2)
+                        for t in (0.0, 2 * math.pi / 3, 4 * math.pi / 3)]
+                rooms = sorted((nearest(c + 0.109 * s, skip0) for s in cand), reverse=True)
+                if rooms[count - 1] > best_room:
+                    best_slots, best_room = cand, rooms[count - 1]
+            slots = best_slots
         skip = {i} | set(nbrs)
         room = [nearest(c + 0.109 * s, skip) for s in slots]
         hs = []
@@ -245,12 +255,12 @@ def _chain(top: _Topology, rng, n_atoms: int, center, radius: float, wall: float
                 r = nearest(q, skip)
                 # the bond also fixes where the last atom's hydrogens go
                 if counts[k - 1] and len(heavy) > 1 and r > room:
-                    r = min(r, slot_room(last, counts[k - 1], top.pos[heavy[-2]], q) + 0.05)
+                    r = min(r, slot_room(last, counts[k - 1], top.pos[heavy[-2]], q) + 0.06)
                 if r > room:
                     p, best, room = q, d, r
-                if r >= 0.25:
+                if r >= 0.30:
                     break
-            if room < 0.22 and k >= 2 and backtracks < 4 * len(counts):
+            if room < 0.27 and k >= 2 and backtracks < 4 * len(counts):
                 # no room here: undo the last `depth` steps and grow again
                 backtracks += 1
                 fails[k] = fails.get(k, 0) + 1
