@@ -177,6 +177,8 @@ def run_ensemble(args):
         e.setVelocitiesToTemperature(r.temperature, r.seed)
         e.run(args.warmup)
         engines.append(e)
+        if len(engines) % 8 == 0:
+            log(f"  {len(engines)} replicas ready ({time.perf_counter() - t:.0f} s)")
     device_synchronize()
     log(f"minimised, thermalised and warmed up {len(engines)} replicas in {time.perf_counter() - t:.1f} s "
         f"({gave_up} minimisations gave up)")

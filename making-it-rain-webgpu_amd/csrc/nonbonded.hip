@@ -78,11 +78,14 @@ struct md_build_ctx {
 // image shells enumerated per axis at most (9 x 9 x 9 images); only boxes
 // not much wider than the cutoff with very sparse clusters come near it
 #define MD_MAX_IMAGE_SHELLS 4
+// candidate j super-clusters a list wave keeps in LDS (beyond: scan all)
+#define MD_LIST_CANDIDATES 1024
 
 struct md_list_ptrs {
   const int64_t *cluster_center;
   const float4 *cluster_half;
   const int64_t *super_origin;
+  const float *super_radius;  // half diagonal of each super-cluster's bounding box (nm)
   int *super_count;   // [nsuper+1] entries per super-cluster, then starts
   int *super_segs;    // [nsuper+1] segments per super-cluster, then starts
   int *entry_j;
@@ -120,6 +123,7 @@ struct md_nb_args {
   int *cell_count, *cell_start, *cell_cursor, *atom_cell, *cell_member;
   int *slot_atom, *atom_slot;
   int64_t *folded, *build_pos, *slot_offset, *cluster_center, *super_origin;
+  float *super_radius;
   float2 *slot_lj;
   int4 *slot_xyzq;
   float4 *cluster_half;
@@ -282,6 +286,7 @@ __global__ void k_nb_clusters(const md_nb_args *__restrict__ T, md_bmap map) {
   const int64_t *folded = A.folded, *build_pos = A.build_pos;
   const float4 *atom_param = A.atom_param;
   int64_t *cluster_center = A.cluster_center, *super_origin = A.super_origin, *slot_offset = A.slot_offset;
+  float *super_radius = A.super_radius;
   float4 *cluster_half = A.cluster_half;
   float2 *slot_lj = A.slot_lj;
   // one thread per cluster; the 8 threads of a super-cluster cooperate
@@ -336,8 +341,16 @@ __global__ void k_nb_clusters(const md_nb_args *__restrict__ T, md_bmap map) {
   int s = k / 8;
   int64_t origin[3];
   for (int d = 0; d < 3; ++d) origin[d] = slo[d] <= shi[d] ? slo[d] + ((shi[d] - slo[d]) >> 1) : 0;
-  if (live && (lane & 7) == 0 && s < c.nsuper)
-    for (int d = 0; d < 3; ++d) super_origin[d * c.nsuper + s] = origin[d];
+  if (live && (lane & 7) == 0 && s < c.nsuper) {
+    float r2 = 0.0f;
+    for (int d = 0; d < 3; ++d) {
+      super_origin[d * c.nsuper + s] = origin[d];
+      float h = 0.5f * md_from_fixed(shi[d] - slo[d]);
+      r2 += h * h;
+    }
+    // rounded up a little: the list test that uses it must stay conservative
+    super_radius[s] = sqrtf(r2) * 1.0001f + 1e-4f;
+  }
   if (!live) return;
   for (int q = 0; q < MD_CLUSTER; ++q) {
     int slot = k * MD_CLUSTER + q;
@@ -474,10 +487,62 @@ __global__ void __launch_bounds__(64) k_nb_list(const md_nb_args *__restrict__ T
     ch[q] = L.cluster_half[k];
   }
   int64_t ox = L.super_origin[S], oy = L.super_origin[c.nsuper + S], oz = L.super_origin[2 * c.nsuper + S];
+  // Candidate j super-clusters SJ >= S, ascending.  A j-cluster of SJ can
+  // reach an i-cluster of S only if the bounding spheres of the two
+  // super-clusters come within the list radius at some image, so SJ is
+  // skipped when its minimum-image centre distance exceeds list radius + both
+  // radii.  The sequential reduction of md_min_image_fixed is the minimum
+  // image in a rectangular box; in a reduced triclinic one the minimum is one
+  // of the 27 images around it, so their smallest distance is used.  A box
+  // that needs image enumeration keeps every SJ.  The j-clusters then go
+  // through the same tests in the same order as a scan of every J >= S's
+  // first cluster, so the list is the one that scan builds.
+  __shared__ int cand[MD_LIST_CANDIDATES];
+  int ncand = 0;
+  if (!enumerate) {
+    const float rs = L.super_radius[S];
+    for (int sb = S; sb < c.nsuper; sb += 64) {
+      const int SJ = sb + lane;
+      bool near = false;
+      if (SJ < c.nsuper) {
+        const float lim = c.list_cutoff + rs + L.super_radius[SJ];
+        if (SJ == S) {
+          near = true;
+        } else {
+          int64_t dx = L.super_origin[SJ] - ox, dy = L.super_origin[c.nsuper + SJ] - oy,
+                  dz = L.super_origin[2 * c.nsuper + SJ] - oz;
+          md_min_image_fixed(dx, dy, dz, c.box);
+          const int k = c.box.rectangular ? 0 : 1;
+          for (int kc = -k; kc <= k; ++kc)
+            for (int kb = -k; kb <= k; ++kb)
+              for (int ka = -k; ka <= k; ++ka) {
+                float fx = md_from_fixed(dx + ka * c.box.fa[0] + kb * c.box.fb[0] + kc * c.box.fc[0]);
+                float fy = md_from_fixed(dy + kb * c.box.fb[1] + kc * c.box.fc[1]);
+                float fz = md_from_fixed(dz + kc * c.box.fc[2]);
+                near |= fx * fx + fy * fy + fz * fz < lim * lim;
+              }
+        }
+      }
+      const uint64_t m = __ballot(near);
+      const int below = __popcll(m & ((uint64_t(1) << lane) - 1));
+      if (near && ncand + below < MD_LIST_CANDIDATES) cand[ncand + below] = SJ;
+      ncand += __popcll(m);
+    }
+    __syncthreads();
+  }
+  // past the candidate store, or enumerating images: scan every J
+  const bool scan_all = enumerate || ncand > MD_LIST_CANDIDATES;
+  const int chunks = scan_all ? (c.nclusters - first + 63) / 64 : (ncand + 7) / 8;
   int written = 0;
   int base_out = PASS == 1 ? L.super_count[S] : 0;
-  for (int jb = first; jb < c.nclusters; jb += 64) {
-    int J = jb + lane;
+  for (int chunk = 0; chunk < chunks; ++chunk) {
+    int J;
+    if (scan_all) {
+      J = first + 64 * chunk + lane;
+    } else {
+      int k = 8 * chunk + (lane >> 3);
+      J = k < ncand ? cand[k] * 8 + (lane & 7) : c.nclusters;
+    }
     bool valid = J < c.nclusters;
     int count = 0;
     if (valid)
@@ -884,6 +949,7 @@ std::vector<int> md_cell_walk(int W, int H, int D) {
 
 struct md_nb_buffers {
   int64_t *cluster_center = nullptr, *super_origin = nullptr;
+  float *super_radius = nullptr;
   int *super_count = nullptr, *super_segs = nullptr;
   int *entry_j = nullptr, *entry_imask = nullptr;
   int4 *entry_shift = nullptr;
@@ -960,6 +1026,7 @@ void md_nb_setup(md_system *s) {
   s->cluster_half = md_alloc<float4>(s, nb.nclusters);
   B.cluster_center = md_alloc<int64_t>(s, 3 * (size_t)nb.nclusters);
   B.super_origin = md_alloc<int64_t>(s, 3 * (size_t)B.nsuper);
+  B.super_radius = md_alloc<float>(s, B.nsuper);
   B.super_count = md_alloc<int>(s, B.nsuper + 1);
   B.super_segs = md_alloc<int>(s, B.nsuper + 1);
   B.totals = md_alloc<int>(s, 4);
@@ -1046,6 +1113,7 @@ static md_nb_args nb_args(md_system *s) {
   L.cluster_center = B.cluster_center;
   L.cluster_half = s->cluster_half;
   L.super_origin = B.super_origin;
+  L.super_radius = B.super_radius;
   L.super_count = B.super_count;
   L.super_segs = B.super_segs;
   L.entry_j = B.entry_j;
@@ -1087,6 +1155,7 @@ static md_nb_args nb_args(md_system *s) {
   a.slot_offset = s->slot_offset;
   a.cluster_center = B.cluster_center;
   a.super_origin = B.super_origin;
+  a.super_radius = B.super_radius;
   a.slot_lj = s->slot_lj;
   a.slot_xyzq = s->slot_xyzq;
   a.cluster_half = s->cluster_half;

@@ -260,7 +260,7 @@ def _chain(top: _Topology, rng, n_atoms: int, center, radius: float, wall: float
                     p, best, room = q, d, r
                 if r >= 0.30:
                     break
-            if room < 0.27 and k >= 2 and backtracks < 4 * len(counts):
+            if room < 0.27 and k >= 2 and backtracks < len(counts) // 2:
                 # no room here: undo the last `depth` steps and grow again
                 backtracks += 1
                 fails[k] = fails.get(k, 0) + 1

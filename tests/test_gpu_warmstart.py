@@ -57,7 +57,7 @@ def test_network_matches_oracle():
 
 @pytest.fixture(scope="module")
 def chain_system():
-    b = synthetic.build(2000, 1500, 2, "cube", 314)
+    b = synthetic.build(600, 2000, 2, "cube", 314)
     e = HipEngine(b.system, b.state, RunOptions(seed=7))
     minimise(e, 100, 10.0)
     e.setVelocitiesToTemperature(300.0, 7)
