@@ -159,11 +159,24 @@ def run_ensemble(args):
 
     t = time.perf_counter()
     workers = max(1, min(16, os.cpu_count() or 1))
-    reps = build_ensemble(args.replicas, workers)
+    if args.ensemble_cache and os.path.exists(args.ensemble_cache):
+        # replicas this script built and saved earlier (same plan and seeds)
+        import pickle
+        with open(args.ensemble_cache, "rb") as f:
+            reps = pickle.load(f)
+        workers = 0
+    else:
+        reps = build_ensemble(args.replicas, workers)
+        if args.ensemble_cache:
+            import pickle
+            with open(args.ensemble_cache, "wb") as f:
+                pickle.dump(reps, f)
     atoms = [r.built.system.particleCount for r in reps]
     total_atoms = int(sum(atoms))
     log(f"built {len(reps)} replicas, {total_atoms} atoms ({min(atoms)}-{max(atoms)}) in "
         f"{time.perf_counter() - t:.1f} s with {workers} workers")
+    if args.build_only:
+        return None
 
     t = time.perf_counter()
     engines = []
@@ -398,6 +411,10 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--max-batch", type=int, default=0,
                     help="steps per captured HIP graph (0: 50; 1: launch every step directly)")
+    ap.add_argument("--ensemble-cache", default="",
+                    help="build the ensemble once into this file and load it on later runs (so a profiled run "
+                         "starts no builder processes)")
+    ap.add_argument("--build-only", action="store_true", help="build (and cache) the ensemble, then stop")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build is single-GPU (asked for --gpus {args.gpus}). "
@@ -405,7 +422,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     result = run_ensemble(args) if args.workload == "ensemble" else run_single(args)
-    print(json.dumps(result), flush=True)
+    if result is not None:
+        print(json.dumps(result), flush=True)
 
 
 if __name__ == "__main__":

@@ -705,14 +705,12 @@ __global__ void k_nb_pack(const md_nb_args *__restrict__ T, md_bmap map, int *se
 // The pair kernel
 // ---------------------------------------------------------------------------
 
-// one pair from lane's i atom to the entry's j atom, its difference d in
-// frame units; returns the F/r factor times the frame unit (so
-// F_i = -fs * d in kJ/mol/nm) and accumulates the pair energy
+// one pair from lane's i atom to the entry's j atom at squared distance r2
+// (nm^2); returns the F/r factor times the frame unit (so F_i = -fs * d with
+// d in frame units gives kJ/mol/nm) and accumulates the pair energy
 template <bool ENERGY>
-__device__ __forceinline__ float md_pair(float dx, float dy, float dz, float qi, float si, float ei, float qj,
-                                         float sj, float ej, bool allowed, const md_pair_args &A, float &en,
-                                         int &count) {
-  float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz)) * MD_FRAME_UNIT2;
+__device__ __forceinline__ float md_pair(float r2, float qi, float si, float ei, float qj, float sj, float ej,
+                                         bool allowed, const md_pair_args &A, float &en, int &count) {
   bool inc = allowed && r2 < A.cutoff2;
   float rinv = __builtin_amdgcn_rsqf(r2);
   float rinv2 = rinv * rinv;
@@ -785,7 +783,11 @@ __device__ __forceinline__ void md_pair_segment(const md_pair_args &A, int seg, 
         bool allowed = (m >> lane) & 1;
         float dx = (float)(int)(xj - (unsigned)xi[q]), dy = (float)(int)(yj - (unsigned)yi[q]),
               dz = (float)(int)(zj - (unsigned)zi[q]);
-        float fs = md_pair<ENERGY>(dx, dy, dz, qi[q], si[q], ei[q], qj, lj.x, lj.y, allowed, A, en, count);
+        float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz)) * MD_FRAME_UNIT2;
+        // a block whose 64 pairs all lie beyond the cutoff (or are masked)
+        // adds exactly zero: skip its arithmetic, wave-uniformly
+        if (__ballot(allowed && r2 < A.cutoff2) == 0) continue;
+        float fs = md_pair<ENERGY>(r2, qi[q], si[q], ei[q], qj, lj.x, lj.y, allowed, A, en, count);
         fx[q] = fmaf(-fs, dx, fx[q]);
         fy[q] = fmaf(-fs, dy, fy[q]);
         fz[q] = fmaf(-fs, dz, fz[q]);
@@ -832,7 +834,10 @@ __device__ __forceinline__ void md_pair_segment(const md_pair_args &A, int seg, 
   if (ENERGY) efix += md_to_fixed(en);
 }
 
-template <bool ENERGY>
+// BATCH only names the instance: a launch over several replicas runs
+// k_nb_pairs<·, true>, one replica <·, false>, the same code, so a kernel
+// trace reports the batched launches on their own.
+template <bool ENERGY, bool BATCH = false>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_nb_pairs(const md_nb_args *__restrict__ T, md_bmap seg_map, int *cursor) {
   const int nseg = seg_map.start[seg_map.R];
   const int lane = threadIdx.x, il = lane >> 3, jl = lane & 7;
@@ -1251,10 +1256,18 @@ void md_nb_pairs(md_group *g, bool energy) {
   hipStream_t st = g->stream;
   k_nb_pack<<<G.slots.total, 256, 0, st>>>(G.table, G.slots.map, const_cast<int *>(G.seg.map.start), G.cursor);
   md_mark(g, MD_PHASE_PAIRS);
-  if (energy)
-    k_nb_pairs<true><<<G.pair_grid[1], 64, 0, st>>>(G.table, G.seg.map, G.cursor);
-  else
-    k_nb_pairs<false><<<G.pair_grid[0], 64, 0, st>>>(G.table, G.seg.map, G.cursor);
+  const bool batch = g->reps.size() > 1;
+  if (energy) {
+    if (batch)
+      k_nb_pairs<true, true><<<G.pair_grid[1], 64, 0, st>>>(G.table, G.seg.map, G.cursor);
+    else
+      k_nb_pairs<true, false><<<G.pair_grid[1], 64, 0, st>>>(G.table, G.seg.map, G.cursor);
+  } else {
+    if (batch)
+      k_nb_pairs<false, true><<<G.pair_grid[0], 64, 0, st>>>(G.table, G.seg.map, G.cursor);
+    else
+      k_nb_pairs<false, false><<<G.pair_grid[0], 64, 0, st>>>(G.table, G.seg.map, G.cursor);
+  }
   MD_HIP(hipGetLastError());
 }
 

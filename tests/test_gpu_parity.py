@@ -13,6 +13,7 @@ import os
 import numpy as np
 import pytest
 
+import edge
 import oracle as O
 from conftest import GOLDEN
 from mdcore import native, synthetic
@@ -27,6 +28,13 @@ ENERGY_REL = 1e-5
 
 def rel_rms(a, b):
     return float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2)))
+
+
+def force_error(forces, of, system, positions, box, o):
+    """Relative RMS against the oracle, pairs within f32 rounding of the
+    cutoff counted either way (tests/edge.py)."""
+    err, _ = edge.rel_rms_edge(forces, of, system, positions, box, o.alpha())
+    return err
 
 
 @pytest.fixture(scope="module")
@@ -106,13 +114,14 @@ def _compare(built, mesh=None):
     s.set_positions(state.positions)
     total, groups, forces = s.evaluate()
     otot, og, of = o.potential(state.positions, system.boxVectors, grid=grid)
-    return s, (total, groups, forces), (otot, og, of)
+    err = force_error(forces, of, system, state.positions, system.boxVectors, o)
+    return s, (total, groups, forces), (otot, og, of), err
 
 
 def test_config0_forces_and_groups(c0):
-    s, (total, groups, forces), (otot, og, of) = _compare(c0)
-    print("config 0 force rel RMS", rel_rms(forces, of), "groups", groups, og)
-    assert rel_rms(forces, of) < FORCE_RMS
+    s, (total, groups, forces), (otot, og, of), err = _compare(c0)
+    print("config 0 force rel RMS", rel_rms(forces, of), "edge-aware", err, "groups", groups, og)
+    assert err < FORCE_RMS
     for k in range(6):
         scale = max(abs(og[k]), 1e-6 * abs(otot), 1.0)
         assert abs(groups[k] - og[k]) / scale < ENERGY_REL, (k, groups[k], og[k])
@@ -120,9 +129,9 @@ def test_config0_forces_and_groups(c0):
 
 
 def test_config1_triclinic_forces_and_groups(c1):
-    s, (total, groups, forces), (otot, og, of) = _compare(c1)
-    print("config 1 force rel RMS", rel_rms(forces, of), "groups", groups, og)
-    assert rel_rms(forces, of) < FORCE_RMS
+    s, (total, groups, forces), (otot, og, of), err = _compare(c1)
+    print("config 1 force rel RMS", rel_rms(forces, of), "edge-aware", err, "groups", groups, og)
+    assert err < FORCE_RMS
     for k in range(6):
         scale = max(abs(og[k]), 1e-6 * abs(otot), 1.0)
         assert abs(groups[k] - og[k]) / scale < ENERGY_REL, (k, groups[k], og[k])
@@ -141,8 +150,9 @@ def test_boundary_frame_counts_each_pair_once(c0):
     s.set_positions(x)
     total, groups, forces = s.evaluate()
     _, og, of = o.potential(x, c0.system.boxVectors, grid=grid)
-    print("boundary frame force rel RMS", rel_rms(forces, of), "direct", groups[3], og[3])
-    assert rel_rms(forces, of) < FORCE_RMS
+    err = force_error(forces, of, c0.system, x, c0.system.boxVectors, o)
+    print("boundary frame force rel RMS", rel_rms(forces, of), "edge-aware", err, "direct", groups[3], og[3])
+    assert err < FORCE_RMS
     assert abs(groups[3] - og[3]) < ENERGY_REL * abs(og[3])
     s.close()
 
@@ -170,7 +180,7 @@ def test_forces_follow_the_oracle_under_translations(c0):
         _, _, of = o.potential(xt, c0.system.boxVectors, grid=grid)
         s.set_positions(xt)
         _, groups, forces = s.evaluate()
-        err = rel_rms(forces, of)
+        err = force_error(forces, of, c0.system, xt, c0.system.boxVectors, o)
         worst = max(worst, err)
         assert err < FORCE_RMS, (k, t, err)
         assert abs(groups[3] - og[3]) < ENERGY_REL * abs(og[3]), (k, groups[3], og[3])
