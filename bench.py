@@ -258,10 +258,15 @@ def run_ensemble(args):
     batch.run(max(args.warmup, 50))
     device_synchronize()
     counts0 = [native.shake_counts(e._native) for e in engines]
+    rebuilds0 = [e._native.list_flags()["rebuilds"] for e in engines]
     t0 = time.perf_counter()
     batch.run(args.steps)
     device_synchronize()
     batch_s = time.perf_counter() - t0
+    rebuilds = sum(e._native.list_flags()["rebuilds"] - r0 for e, r0 in zip(engines, rebuilds0))
+    steps_per_rebuild = len(engines) * args.steps / max(1, rebuilds)
+    log(f"list rebuilds in the timed run: {rebuilds} over {len(engines)} replicas "
+        f"(one per {steps_per_rebuild:.1f} replica-steps)")
     warm_rate = shake_rate(engines, counts0)
     batch_nsday = len(engines) * args.steps * DT_PS * 1e-3 / batch_s * 86400.0
     ms_per_step = batch_s / args.steps * 1e3
@@ -325,6 +330,7 @@ def run_ensemble(args):
                                    "loss_first": loss0, "loss_last": loss1, "seconds": train_s,
                                    "network": "32-64-64-16 tanh MLP, bf16 MFMA 16x16x16, fp32 accumulation"},
             "worst_constraint_rel_deviation": worst_constraint,
+            "steps_per_list_rebuild": steps_per_rebuild,
             "force_rel_rms_vs_oracle": errors,
             "in_cutoff_pairs": int(pairs),
             "arithmetic": "fp32 with Q32.32 int64 accumulation",

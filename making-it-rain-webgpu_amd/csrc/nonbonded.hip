@@ -80,6 +80,8 @@ struct md_build_ctx {
 #define MD_MAX_IMAGE_SHELLS 4
 // candidate j super-clusters a list wave keeps in LDS (beyond: scan all)
 #define MD_LIST_CANDIDATES 1024
+// wavefronts that share one super-cluster's entries in the mask kernel
+#define MD_MASK_WAVES 16
 
 struct md_list_ptrs {
   const int64_t *cluster_center;
@@ -129,6 +131,7 @@ struct md_nb_args {
   float4 *cluster_half;
   const float4 *atom_param;
   const int *excl_start, *excl_list;
+  const uint64_t *excl_window;
   int *totals;
   md_control *ctl;
 };
@@ -407,34 +410,56 @@ __device__ void md_for_each_image(const md_build_ctx &c, int64_t dx, int64_t dy,
       }
 }
 
+// Super-cluster S as a list wave sees it: its clusters' boxes (wave-uniform,
+// indexed only by unrolled constants so they stay in registers), the frame
+// origin and the bounding-sphere radius of the whole super-cluster.
+struct md_super_view {
+  int first, nlocal;
+  int64_t cx[8], cy[8], cz[8];
+  float4 ch[8];
+  int64_t ox, oy, oz;
+  float rs;
+};
+
 // The entries one lane contributes for candidate cluster J: for each passing
 // periodic image, `emit(imask, self, shift)` in a fixed order.
+//
+// Before the eight box tests, an image is tested against S as a whole: if
+// some point p of an i-cluster box and q of the J box lie within the list
+// radius rl, then p is within rs of S's origin O and q within |hJ| of J's
+// centre, so |centre_J - O| < rl + rs + |hJ|.  Failing that bound, no box
+// test can pass, so skipping the image leaves the entries unchanged; the
+// bound is widened by 1e-3 nm against f32 rounding in both tests.
 template <typename F>
-__device__ void md_candidate(const md_build_ctx &c, const md_list_ptrs &L, int first, int nlocal, int J,
-                             bool enumerate, const int *K, const int64_t *cx, const int64_t *cy, const int64_t *cz,
-                             const float4 *ch, F &&emit) {
+__device__ __forceinline__ void md_candidate(const md_build_ctx &c, const md_list_ptrs &L, const md_super_view &V,
+                                             int J, bool enumerate, const int *K, F &&emit) {
   int64_t jx = L.cluster_center[J], jy = L.cluster_center[c.nclusters + J], jz = L.cluster_center[2 * c.nclusters + J];
   float4 jh = L.cluster_half[J];
+  const float reach = c.list_cutoff + V.rs + sqrtf(jh.x * jh.x + jh.y * jh.y + jh.z * jh.z) + 1e-3f;
+  const float reach2 = reach * reach;
   // the image is chosen against S's first cluster and shared by the entry;
   // S's own clusters keep their unshifted frame (see md_for_each_image)
-  int64_t dx = jx - cx[0], dy = jy - cy[0], dz = jz - cz[0];
-  bool own = J < first + 8;
+  int64_t dx = jx - V.cx[0], dy = jy - V.cy[0], dz = jz - V.cz[0];
+  bool own = J < V.first + 8;
   md_for_each_image(c, dx, dy, dz, enumerate, own, K, [&](int64_t ex, int64_t ey, int64_t ez, int code) {
     // the lattice vector that moves J into this image
     int64_t shx = ex - dx, shy = ey - dy, shz = ez - dz;
+    float ox = md_from_fixed(jx + shx - V.ox), oy = md_from_fixed(jy + shy - V.oy), oz = md_from_fixed(jz + shz - V.oz);
+    if (ox * ox + oy * oy + oz * oz >= reach2) return;
     bool zero = shx == 0 && shy == 0 && shz == 0;
     int imask = 0, self = 0;
-    for (int q = 0; q < nlocal; ++q) {
-      int I = first + q;
-      if (J < I) continue;
-      // a cluster with its own images: keep one of each +/- pair
-      if (J == I && !zero && code < 0) continue;
-      float ddx = md_from_fixed(jx + shx - cx[q]);
-      float ddy = md_from_fixed(jy + shy - cy[q]);
-      float ddz = md_from_fixed(jz + shz - cz[q]);
-      if (md_boxes_within(ddx, ddy, ddz, ch[q], jh, c.list_cutoff2)) {
-        imask |= 1 << q;
-        if (J == I && zero) self |= 1 << q;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      int I = V.first + q;
+      // J >= I only; a cluster with its own images keeps one of each +/- pair
+      if (q < V.nlocal && J >= I && !(J == I && !zero && code < 0)) {
+        float ddx = md_from_fixed(jx + shx - V.cx[q]);
+        float ddy = md_from_fixed(jy + shy - V.cy[q]);
+        float ddz = md_from_fixed(jz + shz - V.cz[q]);
+        if (md_boxes_within(ddx, ddy, ddz, V.ch[q], jh, c.list_cutoff2)) {
+          imask |= 1 << q;
+          if (J == I && zero) self |= 1 << q;
+        }
       }
     }
     if (imask) emit(imask, self, shx, shy, shz);
@@ -475,18 +500,23 @@ __global__ void __launch_bounds__(64) k_nb_list(const md_nb_args *__restrict__ T
       K[d] = min(k, MD_MAX_IMAGE_SHELLS);
     }
   if (PASS == 0 && S == 0 && lane == 0 && too_far) ctl->overflow = 2;
-  int first = S * 8;
-  int nlocal = min(8, c.nclusters - first);
-  int64_t cx[8], cy[8], cz[8];
-  float4 ch[8];
+  md_super_view V;
+  const int first = S * 8;
+  V.first = first;
+  V.nlocal = min(8, c.nclusters - first);
+#pragma unroll
   for (int q = 0; q < 8; ++q) {
     int k = min(first + q, c.nclusters - 1);
-    cx[q] = L.cluster_center[k];
-    cy[q] = L.cluster_center[c.nclusters + k];
-    cz[q] = L.cluster_center[2 * c.nclusters + k];
-    ch[q] = L.cluster_half[k];
+    V.cx[q] = L.cluster_center[k];
+    V.cy[q] = L.cluster_center[c.nclusters + k];
+    V.cz[q] = L.cluster_center[2 * c.nclusters + k];
+    V.ch[q] = L.cluster_half[k];
   }
-  int64_t ox = L.super_origin[S], oy = L.super_origin[c.nsuper + S], oz = L.super_origin[2 * c.nsuper + S];
+  const int64_t ox = L.super_origin[S], oy = L.super_origin[c.nsuper + S], oz = L.super_origin[2 * c.nsuper + S];
+  V.ox = ox;
+  V.oy = oy;
+  V.oz = oz;
+  V.rs = L.super_radius[S];
   // Candidate j super-clusters SJ >= S, ascending.  A j-cluster of SJ can
   // reach an i-cluster of S only if the bounding spheres of the two
   // super-clusters come within the list radius at some image, so SJ is
@@ -500,7 +530,7 @@ __global__ void __launch_bounds__(64) k_nb_list(const md_nb_args *__restrict__ T
   __shared__ int cand[MD_LIST_CANDIDATES];
   int ncand = 0;
   if (!enumerate) {
-    const float rs = L.super_radius[S];
+    const float rs = V.rs;
     for (int sb = S; sb < c.nsuper; sb += 64) {
       const int SJ = sb + lane;
       bool near = false;
@@ -546,8 +576,7 @@ __global__ void __launch_bounds__(64) k_nb_list(const md_nb_args *__restrict__ T
     bool valid = J < c.nclusters;
     int count = 0;
     if (valid)
-      md_candidate(c, L, first, nlocal, J, enumerate, K, cx, cy, cz, ch,
-                   [&](int, int, int64_t, int64_t, int64_t) { ++count; });
+      md_candidate(c, L, V, J, enumerate, K, [&](int, int, int64_t, int64_t, int64_t) { ++count; });
     int incl = count;
     for (int off = 1; off < 64; off <<= 1) {
       int v = __shfl_up(incl, off, 64);
@@ -557,20 +586,18 @@ __global__ void __launch_bounds__(64) k_nb_list(const md_nb_args *__restrict__ T
       int at = base_out + written + incl - count;
       int SJ = J / 8;
       int64_t jox = L.super_origin[SJ], joy = L.super_origin[c.nsuper + SJ], joz = L.super_origin[2 * c.nsuper + SJ];
-      md_candidate(c, L, first, nlocal, J, enumerate, K, cx, cy, cz, ch,
-                   [&](int imask, int self, int64_t shx, int64_t shy, int64_t shz) {
-                     if (at < c.entry_capacity) {
-                       L.entry_j[at] = J;
-                       L.entry_imask[at] = imask | (self << 8);
-                       // J's frame origin, moved into this image, in S's frame
-                       bool bad = false;
-                       L.entry_shift[at] = make_int4(md_frame_coord(jox + shx - ox, bad),
-                                                     md_frame_coord(joy + shy - oy, bad),
-                                                     md_frame_coord(joz + shz - oz, bad), 0);
-                       if (bad) ctl->overflow = 3;
-                     }
-                     ++at;
-                   });
+      md_candidate(c, L, V, J, enumerate, K, [&](int imask, int self, int64_t shx, int64_t shy, int64_t shz) {
+        if (at < c.entry_capacity) {
+          L.entry_j[at] = J;
+          L.entry_imask[at] = imask | (self << 8);
+          // J's frame origin, moved into this image, in S's frame
+          bool bad = false;
+          L.entry_shift[at] = make_int4(md_frame_coord(jox + shx - ox, bad), md_frame_coord(joy + shy - oy, bad),
+                                        md_frame_coord(joz + shz - oz, bad), 0);
+          if (bad) ctl->overflow = 3;
+        }
+        ++at;
+      });
     }
     written += __shfl(incl, 63, 64);
   }
@@ -617,48 +644,54 @@ __global__ void k_nb_segments(const md_nb_args *__restrict__ T, md_bmap map) {
   }
 }
 
-// Exclusion masks: one wavefront per entry, lane l = (i = l/8, j = l%8); for
-// each of S's 8 i-clusters a 64-bit mask of the pairs that interact.  Clears
+// Exclusion masks: MD_MASK_WAVES wavefronts per super-cluster S, each taking
+// every MD_MASK_WAVES-th of S's entries, lane l = (i = l/8, j = l%8); for each
+// of S's 8 i-clusters a 64-bit mask of the pairs that interact.  Clears
 // padding, the lower triangle of a cluster with its own zero image, and every
-// excluded pair (the CSR holds both directions; nonbonded.ts:159-182).
+// excluded pair (the CSR holds both directions; nonbonded.ts:159-182).  An
+// exclusion is looked up in the i atom's 64-bit index window, and in the CSR
+// only for the rare atom flagged as having exclusions outside it.
 __global__ void __launch_bounds__(64) k_nb_masks(const md_nb_args *__restrict__ T, md_bmap map) {
   int blk;
-  const int r = md_locate(map, blockIdx.x, blk);
-  const md_nb_args &A = T[r];
+  const md_nb_args &A = md_nb_entry(T, map, blk);
   if (!A.ctl->rebuild_active) return;
-  const int nblk = map.start[r + 1] - map.start[r];
   const md_build_ctx &c = A.c;
   const md_list_ptrs &L = A.L;
+  const int S = blk / MD_MASK_WAVES, part = blk % MD_MASK_WAVES;
+  if (S >= c.nsuper) return;
   const int *slot_atom = A.slot_atom, *excl_start = A.excl_start, *excl_list = A.excl_list;
-  int total = A.totals[0];
-  int lane = threadIdx.x, il = lane >> 3, jl = lane & 7;
-  // one wave per entry, striding over the entries the list holds
-  for (int e = blk; e < total; e += nblk) {
-    // which super-cluster owns entry e: binary search the starts
-    int lo = 0, hi = c.nsuper;
-    while (hi - lo > 1) {
-      int mid = (lo + hi) >> 1;
-      if (L.super_count[mid] <= e) lo = mid; else hi = mid;
-    }
-    int S = lo;
-    int J = L.entry_j[e];
-    int meta = L.entry_imask[e];
-    int jslot = J * MD_CLUSTER + jl;
-    int ja = slot_atom[jslot];
+  const int total = A.totals[0];
+  const int lane = threadIdx.x, il = lane >> 3, jl = lane & 7;
+  int ia[8];
+  uint64_t iw[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int I = S * 8 + q;
+    ia[q] = I < c.nclusters ? slot_atom[I * MD_CLUSTER + il] : -1;
+    iw[q] = ia[q] >= 0 ? A.excl_window[ia[q]] : 0;
+  }
+  const int eb = L.super_count[S], ee = min(L.super_count[S + 1], total);
+  for (int e = eb + part; e < ee; e += MD_MASK_WAVES) {
+    const int J = L.entry_j[e];
+    const int meta = L.entry_imask[e];
+    const int ja = slot_atom[J * MD_CLUSTER + jl];
+#pragma unroll
     for (int q = 0; q < 8; ++q) {
-      int I = S * 8 + q;
-      int islot = I * MD_CLUSTER + il;
-      int ia = I < c.nclusters ? slot_atom[islot] : -1;
-      bool ok = ((meta >> q) & 1) && ia >= 0 && ja >= 0;
+      bool ok = ((meta >> q) & 1) && ia[q] >= 0 && ja >= 0;
       if (ok && ((meta >> (8 + q)) & 1)) ok = jl > il;  // same cluster, zero image
       if (ok) {
-        for (int k = excl_start[ia]; k < excl_start[ia + 1]; ++k)
-          if (excl_list[k] == ja) {
-            ok = false;
-            break;
-          }
+        const int d = ja - ia[q];
+        if (d >= -32 && d < 32 && d != 0) {
+          ok = !((iw[q] >> (d + 32)) & 1);
+        } else if ((iw[q] >> 32) & 1) {
+          for (int k = excl_start[ia[q]]; k < excl_start[ia[q] + 1]; ++k)
+            if (excl_list[k] == ja) {
+              ok = false;
+              break;
+            }
+        }
       }
-      uint64_t m = __ballot(ok);
+      const uint64_t m = __ballot(ok);
       if (lane == 0) L.entry_excl[(size_t)e * 8 + q] = m;
     }
   }
@@ -960,6 +993,7 @@ struct md_nb_buffers {
   int4 *entry_shift = nullptr;
   int4 *segments = nullptr;
   uint64_t *entry_excl = nullptr;
+  uint64_t *excl_window = nullptr;
   int *totals = nullptr;
   int nsuper = 0;
   int entry_capacity = 0, segment_capacity = 0;
@@ -1061,6 +1095,18 @@ void md_nb_setup(md_system *s) {
   if (!s->excl_list.empty())
     MD_HIP(hipMemcpyAsync(s->excl_list_d, s->excl_list.data(), sizeof(int) * s->excl_list.size(),
                           hipMemcpyHostToDevice, s->stream));
+  // each atom's exclusions within 32 indices as a bit window (bit d + 32 for
+  // atom i + d, d in [-32, 31], d != 0); bit 32 marks an atom that also has
+  // exclusions further away, which the mask kernel then looks up in the CSR
+  std::vector<uint64_t> window(s->n, 0);
+  for (int i = 0; i < s->n; ++i)
+    for (int k = s->excl_start[i]; k < s->excl_start[i + 1]; ++k) {
+      const int d = s->excl_list[k] - i;
+      if (d == 0) continue;
+      window[i] |= d >= -32 && d < 32 ? uint64_t(1) << (d + 32) : uint64_t(1) << 32;
+    }
+  B.excl_window = md_alloc<uint64_t>(s, s->n);
+  MD_HIP(hipMemcpyAsync(B.excl_window, window.data(), sizeof(uint64_t) * s->n, hipMemcpyHostToDevice, s->stream));
   MD_HIP(hipStreamSynchronize(s->stream));
 }
 
@@ -1167,6 +1213,7 @@ static md_nb_args nb_args(md_system *s) {
   a.atom_param = s->atom_param;
   a.excl_start = s->excl_start_d;
   a.excl_list = s->excl_list_d;
+  a.excl_window = B.excl_window;
   a.totals = B.totals;
   a.ctl = s->ctl;
   return a;
@@ -1195,7 +1242,7 @@ void md_nb_group_setup(md_group *g) {
     clusters.push_back((s->nb.nclusters + 63) / 64);
     supers.push_back(B.nsuper);
     segments.push_back((B.nsuper + 63) / 64);
-    masks.push_back(std::max(1, std::min(B.entry_capacity, B.nsuper * 32)));
+    masks.push_back(B.nsuper * MD_MASK_WAVES);
     slots.push_back((s->nb.nslots + 255) / 256);
   }
   G->atoms = md_make_launch(g, atoms);
