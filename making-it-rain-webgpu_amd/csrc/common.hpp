@@ -26,7 +26,7 @@
 #define MD_PME_ORDER 5
 #define MD_CLUSTER 8            // atoms per cluster
 #define MD_SUPER 64             // atoms per super-cluster (one wavefront: 8 clusters)
-#define MD_SEGMENT 8            // list entries per wavefront work unit
+#define MD_SEGMENT 8            // list entries per wavefront work unit, at least
 #define MD_MAX_CLUSTER_ATOMS 8  // SHAKE cluster limits (constraint-clusters.ts:68-69)
 #define MD_MAX_CLUSTER_CONSTRAINTS 8
 #define MD_MAX_CONSTRAINT_ITERATIONS 150

@@ -67,6 +67,7 @@ struct md_build_ctx {
   int n, np, ncells, gdim0, gdim1, gdim2;
   int nclusters, nsuper, nslots;
   int entry_capacity, segment_capacity;
+  int seg_len;  // list entries per pair-kernel work unit (MD_SEGMENT or a multiple)
   float list_cutoff2;
   float half_min_width;
   float width[3];  // perpendicular widths along a, b, c
@@ -614,7 +615,7 @@ __global__ void __launch_bounds__(1024) k_nb_list_scan(const md_nb_args *__restr
   const md_list_ptrs &L = A.L;
   const int nsuper = A.c.nsuper, entry_capacity = A.c.entry_capacity, segment_capacity = A.c.segment_capacity;
   int *totals = A.totals;
-  for (int s = threadIdx.x; s < nsuper; s += 1024) L.super_segs[s] = (L.super_count[s] + MD_SEGMENT - 1) / MD_SEGMENT;
+  for (int s = threadIdx.x; s < nsuper; s += 1024) L.super_segs[s] = (L.super_count[s] + A.c.seg_len - 1) / A.c.seg_len;
   __syncthreads();
   md_block_scan(L.super_count, L.super_count, nsuper, nullptr);
   __syncthreads();
@@ -638,9 +639,10 @@ __global__ void k_nb_segments(const md_nb_args *__restrict__ T, md_bmap map) {
   if (S >= A.c.nsuper) return;
   int b = L.super_count[S], e = L.super_count[S + 1];
   int sg = L.super_segs[S];
-  for (int at = b; at < e; at += MD_SEGMENT, ++sg) {
+  const int len = A.c.seg_len;
+  for (int at = b; at < e; at += len, ++sg) {
     if (sg >= segment_capacity) break;
-    L.segments[sg] = make_int4(S, min(at, entry_capacity), min(min(at + MD_SEGMENT, e), entry_capacity), 0);
+    L.segments[sg] = make_int4(S, min(at, entry_capacity), min(min(at + len, e), entry_capacity), 0);
   }
 }
 
@@ -768,8 +770,12 @@ __device__ __forceinline__ float md_pair(float r2, float qi, float si, float ei,
   return fs;
 }
 
-// One segment: up to 8 list entries of super-cluster S against their j
-// clusters.
+// One segment: consecutive list entries of super-cluster S against their j
+// clusters, in units of MD_SEGMENT entries counted from S's first entry.  The
+// i forces and the energy are summed in f32 over one unit and rounded to
+// Q32.32 at its end, then summed as integers over the segment and added once:
+// the rounding never depends on the segment length, so every launch group
+// gives the same bits whatever length it picked.
 template <bool ENERGY>
 __device__ __forceinline__ void md_pair_segment(const md_pair_args &A, int seg, int lane, int il, int jl,
                                                 int64_t &efix, int &count, int &blocks) {
@@ -794,6 +800,9 @@ __device__ __forceinline__ void md_pair_segment(const md_pair_args &A, int seg, 
     fx[q] = fy[q] = fz[q] = 0.0f;
   }
   float en = 0.0f;
+  // lane (il, jl) keeps the i force of atom il of S's cluster q = jl
+  int64_t ax = 0, ay = 0, az = 0;
+  int unit_end = min(eb + MD_SEGMENT, ee);
   for (int e = eb; e < ee; ++e) {
     int J = A.entry_j[e];
     int meta = A.entry_imask[e];
@@ -844,27 +853,42 @@ __device__ __forceinline__ void md_pair_segment(const md_pair_args &A, int seg, 
         md_atomic_add_i64(&A.force[2 * A.np + a], md_to_fixed(gz));
       }
     }
-  }
-  // i forces: sum the 8 lanes that share i (same il), fixed order
+    if (e + 1 == unit_end) {  // wave-uniform: the end of a unit
+      // i forces: sum the 8 lanes that share i (same il), fixed order; every
+      // lane of the butterfly ends with the same sum, lane jl keeps q = jl's
+      float sx = 0.0f, sy = 0.0f, sz = 0.0f;
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    float x = fx[q], y = fy[q], z = fz[q];
-    for (int off = 1; off < 8; off <<= 1) {
-      x += __shfl_xor(x, off, 64);
-      y += __shfl_xor(y, off, 64);
-      z += __shfl_xor(z, off, 64);
-    }
-    if (jl == 0) {
-      int I = S * 8 + q;
-      int a = I < A.nclusters ? A.slot_atom[I * MD_CLUSTER + il] : -1;
-      if (a >= 0) {
-        md_atomic_add_i64(&A.force[a], md_to_fixed(x));
-        md_atomic_add_i64(&A.force[A.np + a], md_to_fixed(y));
-        md_atomic_add_i64(&A.force[2 * A.np + a], md_to_fixed(z));
+      for (int q = 0; q < 8; ++q) {
+        float x = fx[q], y = fy[q], z = fz[q];
+        for (int off = 1; off < 8; off <<= 1) {
+          x += __shfl_xor(x, off, 64);
+          y += __shfl_xor(y, off, 64);
+          z += __shfl_xor(z, off, 64);
+        }
+        if (jl == q) {
+          sx = x;
+          sy = y;
+          sz = z;
+        }
+        fx[q] = fy[q] = fz[q] = 0.0f;
       }
+      ax += md_to_fixed(sx);
+      ay += md_to_fixed(sy);
+      az += md_to_fixed(sz);
+      if (ENERGY) {
+        efix += md_to_fixed(en);
+        en = 0.0f;
+      }
+      unit_end = min(unit_end + MD_SEGMENT, ee);
     }
   }
-  if (ENERGY) efix += md_to_fixed(en);
+  const int I = S * 8 + jl;
+  const int a = I < A.nclusters ? A.slot_atom[I * MD_CLUSTER + il] : -1;
+  if (a >= 0) {
+    md_atomic_add_i64(&A.force[a], ax);
+    md_atomic_add_i64(&A.force[A.np + a], ay);
+    md_atomic_add_i64(&A.force[2 * A.np + a], az);
+  }
 }
 
 // BATCH only names the instance: a launch over several replicas runs
@@ -1156,10 +1180,11 @@ static md_build_ctx build_ctx(md_system *s) {
 }
 
 // one replica's table entry
-static md_nb_args nb_args(md_system *s) {
+static md_nb_args nb_args(md_system *s, int seg_len) {
   auto &B = nbuf(s);
   md_nb_args a{};
   a.c = build_ctx(s);
+  a.c.seg_len = seg_len;
   md_list_ptrs &L = a.L;
   L.cluster_center = B.cluster_center;
   L.cluster_half = s->cluster_half;
@@ -1226,6 +1251,7 @@ struct md_nb_group {
   md_launch seg;                // the pair kernel's segment numbering
   int *cursor = nullptr;
   int pair_grid[2] = {1, 1};
+  int seg_len = MD_SEGMENT;
 };
 
 static md_nb_group &nbg(md_group *g) { return *reinterpret_cast<md_nb_group *>(g->nb); }
@@ -1265,12 +1291,20 @@ void md_nb_group_setup(md_group *g) {
   for (md_system *s : g->reps) capacity += nbuf(s).segment_capacity;
   for (int k = 0; k < 2; ++k)
     G->pair_grid[k] = (int)std::max(1L, std::min(capacity, (long)std::max(1, cus) * std::max(1, per_cu[k])));
+  // Longer segments flush each i-force once per more entries (the flushes
+  // are half of the kernel's 64-bit atomics at 8 entries), but leave fewer
+  // units to share among the resident waves: double the length while at
+  // least 8 units per wave remain, at about 180 entries per super-cluster.
+  double entries = 0.0;
+  for (md_system *s : g->reps) entries += 180.0 * nbuf(s).nsuper;
+  const double waves = (double)G->pair_grid[0];
+  while (G->seg_len < 4 * MD_SEGMENT && entries / (2.0 * G->seg_len) >= 8.0 * waves) G->seg_len *= 2;
   md_nb_group_refresh(g);
 }
 
 void md_nb_group_refresh(md_group *g) {
   std::vector<md_nb_args> host;
-  for (md_system *s : g->reps) host.push_back(nb_args(s));
+  for (md_system *s : g->reps) host.push_back(nb_args(s, nbg(g).seg_len));
   MD_HIP(hipMemcpyAsync(nbg(g).table, host.data(), sizeof(md_nb_args) * host.size(), hipMemcpyHostToDevice,
                         g->stream));
   MD_HIP(hipStreamSynchronize(g->stream));

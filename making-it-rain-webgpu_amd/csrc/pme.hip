@@ -16,6 +16,7 @@
 // stages, twiddles from an f64-computed table); the x-axis pass does forward
 // transform, convolution, energy and inverse transform in one kernel; the
 // z pass that reads the integer grid also clears it for the next step.
+#include <climits>
 #include <cmath>
 #include <vector>
 
@@ -48,6 +49,7 @@ struct md_pme_table {
   int n, np;
   const int64_t *pos;
   const float4 *param;
+  const int *slot_atom;  // the nonbonded list's spatial order, 64-slot super-clusters
   int64_t *grid;
   float2 *cgrid;
   float *phi;
@@ -109,34 +111,103 @@ __device__ __forceinline__ int md_wrap(int i, int n) {
   return i < 0 ? i + n : i;
 }
 
-// one thread per (atom, x-plane of the stencil): 25 integer atomics each
-__global__ void k_pme_spread(const md_pme_table *__restrict__ T, md_bmap map) {
-  int blk;
-  const md_pme_table &E = md_pme_entry(T, map, blk);
+// Charge spreading, one workgroup per 64-slot super-cluster of the nonbonded
+// list (spatially compact: its atoms' stencils overlap), one thread per
+// (slot, x-plane of the stencil).  Each thread's 25 contributions -- the
+// same Q32.32 values as one global atomic each would add -- go into an int64
+// tile in LDS covering the super-cluster's stencil box; the tile is then
+// added to the grid, one global atomic per non-zero point: about a quarter
+// of the global atomics of spreading atom by atom.  Integer sums do not
+// depend on order, so the grid is the same either way.  A super-cluster
+// whose box exceeds the tile spreads straight to the grid.
+#define MD_SPREAD_THREADS (64 * MD_PME_ORDER)
+#define MD_SPREAD_TILE 6144  // int64 points (48 KB of LDS)
+
+__device__ __forceinline__ int md_wrap_near(int d, int n) {
+  d = md_wrap(d, n);
+  return d >= n / 2 ? d - n : d;  // the representative in [-n/2, n/2)
+}
+
+__global__ void __launch_bounds__(MD_SPREAD_THREADS) k_pme_spread(const md_pme_table *__restrict__ T, md_bmap map) {
+  int S;
+  const md_pme_table &E = md_pme_entry(T, map, S);
   const md_pme_args &P = E.P;
   int64_t *grid = E.grid;
-  int t = blk * blockDim.x + threadIdx.x;
-  int i = t / MD_PME_ORDER, a = t - i * MD_PME_ORDER;
-  if (i >= E.n) return;
-  float q = E.param[i].w;
-  if (q == 0.0f) return;
-  int base[3];
-  float w[3];
-  md_mesh_coord(E.pos, E.np, i, P.inv, P.nx, P.ny, P.nz, base, w);
+  __shared__ unsigned long long tile[MD_SPREAD_TILE];
+  __shared__ int box[9];  // reference stencil base, then lowest and highest offsets from it
+  const int t = threadIdx.x, lane = t & 63, a = t >> 6;
+  const int i = E.slot_atom[S * 64 + lane];
+  const float q = i >= 0 ? E.param[i].w : 0.0f;
+  const bool live = q != 0.0f;
+  int base[3] = {0, 0, 0};
+  float w[3] = {0.0f, 0.0f, 0.0f};
+  if (live) md_mesh_coord(E.pos, E.np, i, P.inv, P.nx, P.ny, P.nz, base, w);
+  const int n[3] = {P.nx, P.ny, P.nz};
+  // the first wave (plane 0 of every slot) finds the stencil box: offsets of
+  // each base from the first live slot's, taken to the nearest periodic
+  // representative (any representative lands on the same grid points; the
+  // nearest keeps the box small)
+  if (a == 0) {
+    const uint64_t m = __ballot(live);
+    const int first = m ? __ffsll((unsigned long long)m) - 1 : 0;
+    for (int d = 0; d < 3; ++d) {
+      const int ref = __shfl(base[d], first, 64);
+      const int off = live ? md_wrap_near(base[d] - ref, n[d]) : 0;
+      int lo = live ? off : INT_MAX, hi = live ? off : INT_MIN;
+      for (int k = 32; k > 0; k >>= 1) {
+        lo = min(lo, __shfl_xor(lo, k, 64));
+        hi = max(hi, __shfl_xor(hi, k, 64));
+      }
+      if (lane == 0) {
+        box[d] = ref;
+        box[3 + d] = m ? lo : 0;
+        box[6 + d] = m ? hi : -MD_PME_ORDER;  // no charge: an empty box
+      }
+    }
+  }
+  __syncthreads();
+  int ext[3], off[3];
+  for (int d = 0; d < 3; ++d) {
+    ext[d] = box[6 + d] - box[3 + d] + MD_PME_ORDER;
+    off[d] = live ? md_wrap_near(base[d] - box[d], n[d]) - box[3 + d] : 0;
+  }
+  const int volume = ext[0] * ext[1] * ext[2];
+  if (volume == 0) return;  // no charged atom in this super-cluster (uniform)
   float tx[5], ty[5], tz[5];
   md_bspline5(w[0], tx, nullptr);
   md_bspline5(w[1], ty, nullptr);
   md_bspline5(w[2], tz, nullptr);
-  int gx = md_wrap(base[0] + a, P.nx);
-  float wx = q * tx[a];
-  for (int b = 0; b < 5; ++b) {
-    int gy = md_wrap(base[1] + b, P.ny);
-    float wxy = wx * ty[b];
-    int64_t *row = grid + ((size_t)gx * P.ny + gy) * P.nz;
-    for (int c = 0; c < 5; ++c) {
-      int gz = md_wrap(base[2] + c, P.nz);
-      md_atomic_add_i64(&row[gz], md_to_fixed(wxy * tz[c]));
+  const float wx = q * tx[a];
+  if (volume > MD_SPREAD_TILE) {
+    if (!live) return;
+    const int gx = md_wrap(base[0] + a, P.nx);
+    for (int b = 0; b < 5; ++b) {
+      const int gy = md_wrap(base[1] + b, P.ny);
+      const float wxy = wx * ty[b];
+      int64_t *row = grid + ((size_t)gx * P.ny + gy) * P.nz;
+      for (int c = 0; c < 5; ++c) md_atomic_add_i64(&row[md_wrap(base[2] + c, P.nz)], md_to_fixed(wxy * tz[c]));
     }
+    return;
+  }
+  for (int k = t; k < volume; k += MD_SPREAD_THREADS) tile[k] = 0;
+  __syncthreads();
+  if (live) {
+    const int plane = (off[0] + a) * ext[1];
+    for (int b = 0; b < 5; ++b) {
+      const float wxy = wx * ty[b];
+      unsigned long long *row = tile + (size_t)(plane + off[1] + b) * ext[2] + off[2];
+      for (int c = 0; c < 5; ++c) atomicAdd(&row[c], (unsigned long long)md_to_fixed(wxy * tz[c]));
+    }
+  }
+  __syncthreads();
+  const int yz = ext[1] * ext[2];
+  for (int k = t; k < volume; k += MD_SPREAD_THREADS) {
+    const long long v = (long long)tile[k];
+    if (v == 0) continue;
+    const int lx = k / yz, r = k - lx * yz, ly = r / ext[2], lz = r - ly * ext[2];
+    const int gx = md_wrap(box[0] + box[3] + lx, P.nx), gy = md_wrap(box[1] + box[4] + ly, P.ny),
+              gz = md_wrap(box[2] + box[5] + lz, P.nz);
+    md_atomic_add_i64(grid + ((size_t)gx * P.ny + gy) * P.nz + gz, v);
   }
 }
 
@@ -529,6 +600,7 @@ static md_pme_table pme_table(md_system *s) {
   E.np = s->np;
   E.pos = s->pos;
   E.param = s->atom_param;
+  E.slot_atom = s->slot_atom;
   E.grid = s->pme_grid;
   E.cgrid = s->pme_cgrid;
   E.phi = s->pme_phi;
@@ -553,7 +625,7 @@ void md_pme_group_setup(md_group *g) {
   for (md_system *s : g->reps) {
     bool on = s->has_nonbonded;
     int nx = s->pme_nx, ny = s->pme_ny, nz = s->pme_nz, kzb = (s->pme_nzc + PME_BATCH - 1) / PME_BATCH;
-    spread.push_back(on ? (s->n * MD_PME_ORDER + 255) / 256 : 0);
+    spread.push_back(on ? s->nb.nslots / 64 : 0);
     zrows.push_back(on ? (nx * ny + PME_BATCH - 1) / PME_BATCH : 0);
     ylines.push_back(on ? nx * kzb : 0);
     xlines.push_back(on ? ny * kzb : 0);
@@ -600,7 +672,7 @@ void md_pme(md_group *g, bool energy) {
   if (G.spread.total == 0) return;
   hipStream_t st = g->stream;
   const md_pme_table *T = G.table;
-  k_pme_spread<<<G.spread.total, 256, 0, st>>>(T, G.spread.map);
+  k_pme_spread<<<G.spread.total, MD_SPREAD_THREADS, 0, st>>>(T, G.spread.map);
   md_mark(g, MD_PHASE_FFT);
   k_pme_z_forward<<<G.zrows.total, 256, G.lz, st>>>(T, G.zrows.map);
   k_pme_y<<<G.ylines.total, 256, G.ly, st>>>(T, G.ylines.map, -1.0f);
