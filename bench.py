@@ -363,6 +363,50 @@ def run_ensemble(args):
     return result
 
 
+def run_traffic_probe(args):
+    """A short run for hardware-counter passes (rocprofv3 --pmc): the ensemble's
+    replicas as built (no minimisation or equilibration, 0.5 fs steps at 10/ps
+    friction so the raw geometry stays bounded), a few batched steps, then one
+    device-to-device copy of COPY_BYTES whose counter reading scales the
+    counter to bytes."""
+    import ctypes
+
+    from mdcore.engine import HipBatch, HipEngine, RunOptions
+
+    if args.ensemble_cache and os.path.exists(args.ensemble_cache):
+        import pickle
+        with open(args.ensemble_cache, "rb") as f:
+            reps = pickle.load(f)
+    else:
+        reps = build_ensemble(args.replicas, max(1, min(16, os.cpu_count() or 1)))
+    engines = [HipEngine(r.built.system, r.built.state,
+                         RunOptions(timeStep=0.0005, friction=10.0, temperature=r.temperature,
+                                    constraintTolerance=1e-6, seed=r.seed, maxBatch=1)) for r in reps]
+    batch = HipBatch(engines)
+    batch.run(args.traffic_probe)
+    device_synchronize()
+    log(f"traffic probe: {args.traffic_probe} batched steps of {len(engines)} replicas")
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(COPY_BYTES)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(src, 1, ctypes.c_size_t(COPY_BYTES)) != 0:
+        raise RuntimeError("hipMemset failed")
+    if hip.hipMemcpy(dst, src, ctypes.c_size_t(COPY_BYTES), 3) != 0:  # device to device
+        raise RuntimeError("hipMemcpy failed")
+    device_synchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    log(f"traffic probe: copied {COPY_BYTES} bytes device to device")
+    batch.destroy()
+    for e in engines:
+        e.destroy()
+
+
+COPY_BYTES = 1 << 28
+
+
 def run_single(args):
     from mdcore import synthetic
     from mdcore.engine import HipEngine, RunOptions, minimise
@@ -436,12 +480,18 @@ def main():
                     help="build the ensemble once into this file and load it on later runs (so a profiled run "
                          "starts no builder processes)")
     ap.add_argument("--build-only", action="store_true", help="build (and cache) the ensemble, then stop")
+    ap.add_argument("--traffic-probe", type=int, default=0, metavar="STEPS",
+                    help="instead of the benchmark: STEPS batched steps of the ensemble and one copy of known size, "
+                         "for rocprofv3 --pmc passes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build is single-GPU (asked for --gpus {args.gpus}). "
               "Independent replicas would shard across GPUs with no communication between them.",
               file=sys.stderr)
         sys.exit(2)
+    if args.traffic_probe > 0:
+        run_traffic_probe(args)
+        return
     result = run_ensemble(args) if args.workload == "ensemble" else run_single(args)
     if result is not None:
         print(json.dumps(result), flush=True)

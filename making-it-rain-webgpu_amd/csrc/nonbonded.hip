@@ -106,13 +106,16 @@ struct md_pair_args {
   const uint64_t *entry_excl;
   const int4 *slot_xyzq;  // frame coordinates (MD_FRAME_UNIT units) and the charge's bits
   const float2 *slot_lj;
-  const int *slot_atom;
   const int *totals;
-  int64_t *force;
+  // Q32.32 forces by slot, [3][nslots]: a cluster's 8 atoms are 8 adjacent
+  // words, so the 8 or 64 lanes of one atomic instruction hit one or a few
+  // cache lines instead of as many random atoms; k_nb_forces moves them to
+  // the atoms
+  int64_t *slot_force;
   int64_t *energy;
   int64_t *census;  // in-cutoff pairs (md_control)
   int64_t *blocks;  // 8x8 cluster-pair blocks evaluated (md_control)
-  int np, nclusters;
+  int nslots, nclusters;
   float cutoff2, alpha, beta, exp_scale;
 };
 
@@ -133,6 +136,7 @@ struct md_nb_args {
   const float4 *atom_param;
   const int *excl_start, *excl_list;
   const uint64_t *excl_window;
+  int64_t *force;  // the atoms' Q32.32 forces, [3][np]
   int *totals;
   md_control *ctl;
 };
@@ -575,9 +579,18 @@ __global__ void __launch_bounds__(64) k_nb_list(const md_nb_args *__restrict__ T
       J = k < ncand ? cand[k] * 8 + (lane & 7) : c.nclusters;
     }
     bool valid = J < c.nclusters;
-    int count = 0;
+    // the count, and the entry itself when there is just one (always, without
+    // image enumeration): only several images need a second pass to write
+    int count = 0, one_mask = 0;
+    int64_t one_shift[3] = {0, 0, 0};
     if (valid)
-      md_candidate(c, L, V, J, enumerate, K, [&](int, int, int64_t, int64_t, int64_t) { ++count; });
+      md_candidate(c, L, V, J, enumerate, K, [&](int imask, int self, int64_t shx, int64_t shy, int64_t shz) {
+        ++count;
+        one_mask = imask | (self << 8);
+        one_shift[0] = shx;
+        one_shift[1] = shy;
+        one_shift[2] = shz;
+      });
     int incl = count;
     for (int off = 1; off < 64; off <<= 1) {
       int v = __shfl_up(incl, off, 64);
@@ -587,10 +600,10 @@ __global__ void __launch_bounds__(64) k_nb_list(const md_nb_args *__restrict__ T
       int at = base_out + written + incl - count;
       int SJ = J / 8;
       int64_t jox = L.super_origin[SJ], joy = L.super_origin[c.nsuper + SJ], joz = L.super_origin[2 * c.nsuper + SJ];
-      md_candidate(c, L, V, J, enumerate, K, [&](int imask, int self, int64_t shx, int64_t shy, int64_t shz) {
+      auto write = [&](int meta, int64_t shx, int64_t shy, int64_t shz) {
         if (at < c.entry_capacity) {
           L.entry_j[at] = J;
-          L.entry_imask[at] = imask | (self << 8);
+          L.entry_imask[at] = meta;
           // J's frame origin, moved into this image, in S's frame
           bool bad = false;
           L.entry_shift[at] = make_int4(md_frame_coord(jox + shx - ox, bad), md_frame_coord(joy + shy - oy, bad),
@@ -598,7 +611,13 @@ __global__ void __launch_bounds__(64) k_nb_list(const md_nb_args *__restrict__ T
           if (bad) ctl->overflow = 3;
         }
         ++at;
-      });
+      };
+      if (count == 1)
+        write(one_mask, one_shift[0], one_shift[1], one_shift[2]);
+      else
+        md_candidate(c, L, V, J, enumerate, K, [&](int imask, int self, int64_t shx, int64_t shy, int64_t shz) {
+          write(imask | (self << 8), shx, shy, shz);
+        });
     }
     written += __shfl(incl, 63, 64);
   }
@@ -736,6 +755,24 @@ __global__ void k_nb_pack(const md_nb_args *__restrict__ T, md_bmap map, int *se
   if (bad) A.ctl->overflow = 3;
 }
 
+// After the pair kernel: each slot's force onto its atom, and the slot cleared
+// for the next step.  Kernels on one stream run in order and a slot is the
+// only one holding its atom, so a plain read-add-write is enough.
+__global__ void k_nb_forces(const md_nb_args *__restrict__ T, md_bmap map) {
+  int blk;
+  const md_nb_args &A = md_nb_entry(T, map, blk);
+  const int nslots = A.c.nslots, np = A.c.np;
+  const int s = blk * blockDim.x + threadIdx.x;
+  if (s >= nslots) return;
+  int64_t *slot_force = A.P.slot_force;
+  const int a = A.slot_atom[s];
+  for (int d = 0; d < 3; ++d) {
+    const int64_t v = slot_force[d * nslots + s];
+    slot_force[d * nslots + s] = 0;
+    if (a >= 0) A.force[d * np + a] += v;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // The pair kernel
 // ---------------------------------------------------------------------------
@@ -845,13 +882,11 @@ __device__ __forceinline__ void md_pair_segment(const md_pair_args &A, int seg, 
       gy += __shfl_xor(gy, off, 64);
       gz += __shfl_xor(gz, off, 64);
     }
-    if (lane < 8) {
-      int a = A.slot_atom[J * MD_CLUSTER + lane];
-      if (a >= 0) {
-        md_atomic_add_i64(&A.force[a], md_to_fixed(gx));
-        md_atomic_add_i64(&A.force[A.np + a], md_to_fixed(gy));
-        md_atomic_add_i64(&A.force[2 * A.np + a], md_to_fixed(gz));
-      }
+    if (lane < 8) {  // padding slots collect zeros and are never moved to an atom
+      const int js = J * MD_CLUSTER + lane;
+      md_atomic_add_i64(&A.slot_force[js], md_to_fixed(gx));
+      md_atomic_add_i64(&A.slot_force[A.nslots + js], md_to_fixed(gy));
+      md_atomic_add_i64(&A.slot_force[2 * A.nslots + js], md_to_fixed(gz));
     }
     if (e + 1 == unit_end) {  // wave-uniform: the end of a unit
       // i forces: sum the 8 lanes that share i (same il), fixed order; every
@@ -882,13 +917,11 @@ __device__ __forceinline__ void md_pair_segment(const md_pair_args &A, int seg, 
       unit_end = min(unit_end + MD_SEGMENT, ee);
     }
   }
-  const int I = S * 8 + jl;
-  const int a = I < A.nclusters ? A.slot_atom[I * MD_CLUSTER + il] : -1;
-  if (a >= 0) {
-    md_atomic_add_i64(&A.force[a], ax);
-    md_atomic_add_i64(&A.force[A.np + a], ay);
-    md_atomic_add_i64(&A.force[2 * A.np + a], az);
-  }
+  // 64 consecutive slots: S's cluster jl, atom il
+  const int is = (S * 8 + jl) * MD_CLUSTER + il;
+  md_atomic_add_i64(&A.slot_force[is], ax);
+  md_atomic_add_i64(&A.slot_force[A.nslots + is], ay);
+  md_atomic_add_i64(&A.slot_force[2 * A.nslots + is], az);
 }
 
 // BATCH only names the instance: a launch over several replicas runs
@@ -1018,6 +1051,7 @@ struct md_nb_buffers {
   int4 *segments = nullptr;
   uint64_t *entry_excl = nullptr;
   uint64_t *excl_window = nullptr;
+  int64_t *slot_force = nullptr;
   int *totals = nullptr;
   int nsuper = 0;
   int entry_capacity = 0, segment_capacity = 0;
@@ -1130,6 +1164,7 @@ void md_nb_setup(md_system *s) {
       window[i] |= d >= -32 && d < 32 ? uint64_t(1) << (d + 32) : uint64_t(1) << 32;
     }
   B.excl_window = md_alloc<uint64_t>(s, s->n);
+  B.slot_force = md_alloc<int64_t>(s, 3 * (size_t)nb.nslots);
   MD_HIP(hipMemcpyAsync(B.excl_window, window.data(), sizeof(uint64_t) * s->n, hipMemcpyHostToDevice, s->stream));
   MD_HIP(hipStreamSynchronize(s->stream));
 }
@@ -1205,13 +1240,12 @@ static md_nb_args nb_args(md_system *s, int seg_len) {
   P.entry_excl = B.entry_excl;
   P.slot_xyzq = s->slot_xyzq;
   P.slot_lj = s->slot_lj;
-  P.slot_atom = s->slot_atom;
   P.totals = B.totals;
-  P.force = s->force;
+  P.slot_force = B.slot_force;
   P.energy = s->energy;
   P.census = &s->ctl->pair_census;
   P.blocks = &s->ctl->pair_blocks;
-  P.np = s->np;
+  P.nslots = s->nb.nslots;
   P.nclusters = s->nb.nclusters;
   P.cutoff2 = s->nb.cutoff2;
   P.alpha = (float)s->alpha;
@@ -1239,6 +1273,7 @@ static md_nb_args nb_args(md_system *s, int seg_len) {
   a.excl_start = s->excl_start_d;
   a.excl_list = s->excl_list_d;
   a.excl_window = B.excl_window;
+  a.force = s->force;
   a.totals = B.totals;
   a.ctl = s->ctl;
   return a;
@@ -1349,6 +1384,7 @@ void md_nb_pairs(md_group *g, bool energy) {
     else
       k_nb_pairs<false, false><<<G.pair_grid[0], 64, 0, st>>>(G.table, G.seg.map, G.cursor);
   }
+  k_nb_forces<<<G.slots.total, 256, 0, st>>>(G.table, G.slots.map);
   MD_HIP(hipGetLastError());
 }
 
