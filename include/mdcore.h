@@ -240,7 +240,7 @@ md_err md_list_flags_get(md_system *sys, md_list_flags *flags);
  * per step of each phase (MD_PHASE_COUNT entries). */
 enum {
   MD_PHASE_PROLOGUE = 0, /* step counter, rebuild latch */
-  MD_PHASE_BUILD = 1,    /* cell sort + cluster-pair list (gated on the device) */
+  MD_PHASE_BUILD = 1,    /* cell sort, clusters, bounding boxes (gated on the device) */
   MD_PHASE_PACK = 2,     /* cluster-frame f32 coordinates */
   MD_PHASE_PAIRS = 3,    /* LJ + erfc-Ewald pair kernel */
   MD_PHASE_BONDED = 4,   /* bonds, angles, torsions, exceptions, restraints */
@@ -248,7 +248,10 @@ enum {
   MD_PHASE_FFT = 6,      /* PME forward FFT, convolution, inverse FFT */
   MD_PHASE_GATHER = 7,   /* PME force interpolation */
   MD_PHASE_INTEGRATE = 8,/* LFMiddle/Verlet + SETTLE/SHAKE/RATTLE + COM */
-  MD_PHASE_COUNT = 9
+  MD_PHASE_SEARCH = 9,   /* cluster-pair search, list starts, segments (gated) */
+  MD_PHASE_MASKS = 10,   /* exclusion masks and the list by j-cluster (gated) */
+  MD_PHASE_FORCES = 11,  /* pair forces from slots and list entries onto the atoms */
+  MD_PHASE_COUNT = 12
 };
 md_err md_profile_steps(md_system *sys, int32_t steps, double *ms);
 const char *md_phase_name(int32_t phase);

@@ -949,8 +949,9 @@ md_err md_list_flags_get(md_system *s, md_list_flags *flags) {
 }
 
 const char *md_phase_name(int32_t phase) {
-  static const char *names[MD_PHASE_COUNT] = {"prologue", "list_build", "pack",   "pairs",    "bonded",
-                                              "pme_spread", "pme_fft_convolve", "pme_gather", "integrate"};
+  static const char *names[MD_PHASE_COUNT] = {"prologue",   "list_build", "pack",        "pairs",
+                                              "bonded",     "pme_spread", "pme_fft_convolve", "pme_gather",
+                                              "integrate",  "list_search", "list_masks", "pair_forces"};
   return phase >= 0 && phase < MD_PHASE_COUNT ? names[phase] : "";
 }
 

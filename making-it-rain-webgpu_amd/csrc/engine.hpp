@@ -40,7 +40,8 @@ struct md_control {
   uint32_t overflow;               // the cluster-pair list exceeded its capacity
   int64_t rebuilds;
   uint32_t max_disp2_bits[2];      // [0] worst squared displacement since the build
-  uint32_t cluster_reach_bits[2];  // largest cluster radius, super-cluster reach
+  uint32_t cluster_radius_bits;    // largest cluster bounding-box half diagonal (f32 bits)
+  uint32_t super_radius_bits;      // largest super-cluster radius (f32 bits)
   int64_t momentum[2][3];          // sum m v in Q32.32, by step parity
   int64_t pair_census;             // in-cutoff pairs, when counted
   uint32_t force_pending;          // the host asks for a rebuild at the next step

@@ -21,13 +21,13 @@
 //     (i-cluster, j-cluster) block.  i-forces stay in registers across a
 //     segment of entries; j-forces are reduced across the 8 lanes sharing a j
 //     with cross-lane shuffles once per entry.  No LDS, no per-pair shuffle.
-//   * Positions reach the kernel as f32 offsets from the super-cluster's
-//     fixed-point frame origin, differenced exactly in Q32.32 first, so an
-//     absolute coordinate is never rounded (the reference's wrap-then-convert
-//     rule, fixed64.wgsl:166-216).
-//   * Every f32 partial sum covers one fixed list segment or entry and is
-//     converted to Q32.32 before it is added anywhere, so two identical runs
-//     are bit-identical whatever order the hardware schedules waves in.
+//   * Positions reach the kernel as int32 coordinates in 2^-26 nm from the
+//     super-cluster's fixed-point frame origin, so a pair difference is exact
+//     integer arithmetic, rounded once when it becomes f32 (the reference's
+//     difference-then-convert rule, fixed64.wgsl:166-216).
+//   * Every f32 partial sum covers one list entry or one unit of 8 entries and
+//     is converted to Q32.32 before it is added anywhere, so two identical
+//     runs are bit-identical whatever order the hardware schedules waves in.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -96,6 +96,9 @@ struct md_list_ptrs {
   int4 *entry_shift;    // J's frame origin in S's frame, MD_FRAME_UNIT units
   int4 *segments;     // (S, begin, end, 0)
   uint64_t *entry_excl;  // [capacity][8]
+  // the transposed list: for j-cluster J, the entries naming it are
+  // j_entries[j_start[J] .. j_start[J+1]) (any order); j_count counts them
+  int *j_count, *j_start, *j_entries;
 };
 
 struct md_pair_args {
@@ -112,6 +115,9 @@ struct md_pair_args {
   // cache lines instead of as many random atoms; k_nb_forces moves them to
   // the atoms
   int64_t *slot_force;
+  // per list entry, the f32 force on each of J's 8 atoms, [entry][x|y|z][8]:
+  // one plain store per entry instead of 24 atomics; k_nb_forces sums them
+  float *j_force;
   int64_t *energy;
   int64_t *census;  // in-cutoff pairs (md_control)
   int64_t *blocks;  // 8x8 cluster-pair blocks evaluated (md_control)
@@ -248,8 +254,8 @@ __global__ void __launch_bounds__(1024) k_nb_scan_cells(const md_nb_args *__rest
   if (threadIdx.x == 0) {
     ctl->rebuilds += 1;
     ctl->max_disp2_bits[0] = 0;
-    ctl->cluster_reach_bits[0] = 0;
-    ctl->cluster_reach_bits[1] = 0;
+    ctl->cluster_radius_bits = 0;
+    ctl->super_radius_bits = 0;
   }
 }
 
@@ -320,20 +326,11 @@ __global__ void k_nb_clusters(const md_nb_args *__restrict__ T, md_bmap map) {
     half[d] = live && lo[d] <= hi[d] ? md_from_fixed(hi[d] - lo[d]) * 0.5f : 0.0f;
   }
   float radius = sqrtf(half[0] * half[0] + half[1] * half[1] + half[2] * half[2]);
-  // reach of the super-cluster measured from its first cluster's centre
-  float reach = 0.0f;
-  {
-    int64_t c0[3];
-    for (int d = 0; d < 3; ++d) c0[d] = __shfl(cen[d], lane & ~7, 64);
-    float ddx = md_from_fixed(cen[0] - c0[0]), ddy = md_from_fixed(cen[1] - c0[1]),
-          ddz = md_from_fixed(cen[2] - c0[2]);
-    reach = live ? sqrtf(ddx * ddx + ddy * ddy + ddz * ddz) + radius : 0.0f;
-  }
   if (live) {
     for (int d = 0; d < 3; ++d) cluster_center[d * c.nclusters + k] = cen[d];
     cluster_half[k] = make_float4(half[0], half[1], half[2], 0.0f);
-    atomicMax(&ctl->cluster_reach_bits[0], __float_as_uint(radius));
-    atomicMax(&ctl->cluster_reach_bits[1], __float_as_uint(reach));
+    A.L.j_count[k] = 0;
+    atomicMax(&ctl->cluster_radius_bits, __float_as_uint(radius));
   }
   // super-cluster frame: the bounding box centre over its 8 clusters
   int64_t slo[3], shi[3];
@@ -358,6 +355,7 @@ __global__ void k_nb_clusters(const md_nb_args *__restrict__ T, md_bmap map) {
     }
     // rounded up a little: the list test that uses it must stay conservative
     super_radius[s] = sqrtf(r2) * 1.0001f + 1e-4f;
+    atomicMax(&ctl->super_radius_bits, __float_as_uint(super_radius[s]));
   }
   if (!live) return;
   for (int q = 0; q < MD_CLUSTER; ++q) {
@@ -442,9 +440,9 @@ __device__ __forceinline__ void md_candidate(const md_build_ctx &c, const md_lis
   float4 jh = L.cluster_half[J];
   const float reach = c.list_cutoff + V.rs + sqrtf(jh.x * jh.x + jh.y * jh.y + jh.z * jh.z) + 1e-3f;
   const float reach2 = reach * reach;
-  // the image is chosen against S's first cluster and shared by the entry;
-  // S's own clusters keep their unshifted frame (see md_for_each_image)
-  int64_t dx = jx - V.cx[0], dy = jy - V.cy[0], dz = jz - V.cz[0];
+  // the image is chosen against S's origin and shared by the entry; S's own
+  // clusters keep their unshifted frame (see md_for_each_image)
+  int64_t dx = jx - V.ox, dy = jy - V.oy, dz = jz - V.oz;
   bool own = J < V.first + 8;
   md_for_each_image(c, dx, dy, dz, enumerate, own, K, [&](int64_t ex, int64_t ey, int64_t ez, int code) {
     // the lattice vector that moves J into this image
@@ -484,27 +482,6 @@ __global__ void __launch_bounds__(64) k_nb_list(const md_nb_args *__restrict__ T
   const md_list_ptrs &L = A.L;
   int lane = threadIdx.x;
   if (S >= c.nsuper) return;
-  float reach = __uint_as_float(ctl->cluster_reach_bits[0]) + __uint_as_float(ctl->cluster_reach_bits[1]);
-  // Every image of a j-cluster that can come within the list radius of S lies
-  // within R of S's first cluster.  Below half the narrowest width that is the
-  // minimum image alone.  Otherwise an image K_i + 1 or more lattice steps
-  // along axis i has fractional coordinate at least K_i + 1 - f_i there (f_i
-  // bounds the difference it was stepped from: `fmax` for a reduced one,
-  // reach/w_i for S's own clusters, which are not reduced), so it lies at least
-  // (K_i + 1 - f_i) w_i away: K_i = ceil(R / w_i + f_i - 1) shells suffice.
-  // (An own cluster's centre is within `reach` of the first cluster's.)
-  const float R = c.list_cutoff + reach;
-  const bool enumerate = R > c.half_min_width;
-  int K[3] = {1, 1, 1};
-  bool too_far = false;
-  if (enumerate)
-    for (int d = 0; d < 3; ++d) {
-      float f = fmaxf(c.fmax[d], reach / c.width[d]);
-      int k = max(1, (int)ceilf(R / c.width[d] + f - 1.0f));
-      too_far |= k > MD_MAX_IMAGE_SHELLS;
-      K[d] = min(k, MD_MAX_IMAGE_SHELLS);
-    }
-  if (PASS == 0 && S == 0 && lane == 0 && too_far) ctl->overflow = 2;
   md_super_view V;
   const int first = S * 8;
   V.first = first;
@@ -522,20 +499,54 @@ __global__ void __launch_bounds__(64) k_nb_list(const md_nb_args *__restrict__ T
   V.oy = oy;
   V.oz = oz;
   V.rs = L.super_radius[S];
+  // Every image of a j-cluster that can come within the list radius of one of
+  // S's clusters lies within R = list radius + largest cluster radius + r_S of
+  // S's origin (some point of an i-cluster box, within r_S of the origin, comes
+  // within the list radius of some point of the J box).  Below half the
+  // narrowest width that is the minimum image alone.  Otherwise an image K_i +
+  // 1 or more lattice steps along axis i has fractional coordinate at least
+  // K_i + 1 - f_i there (f_i bounds the difference it was stepped from: `fmax`
+  // for a reduced one, r_S/w_i for S's own clusters, which are not reduced),
+  // so it lies at least (K_i + 1 - f_i) w_i away: K_i = ceil(R / w_i + f_i -
+  // 1) shells suffice.
+  const float R = c.list_cutoff + __uint_as_float(ctl->cluster_radius_bits) + V.rs;
+  const bool enumerate = R > c.half_min_width;
+  int K[3] = {1, 1, 1};
+  bool too_far = false;
+  if (enumerate)
+    for (int d = 0; d < 3; ++d) {
+      float f = fmaxf(c.fmax[d], V.rs / c.width[d]);
+      int k = max(1, (int)ceilf(R / c.width[d] + f - 1.0f));
+      too_far |= k > MD_MAX_IMAGE_SHELLS;
+      K[d] = min(k, MD_MAX_IMAGE_SHELLS);
+    }
+  if (PASS == 0 && lane == 0 && too_far) ctl->overflow = 2;
   // Candidate j super-clusters SJ >= S, ascending.  A j-cluster of SJ can
   // reach an i-cluster of S only if the bounding spheres of the two
   // super-clusters come within the list radius at some image, so SJ is
-  // skipped when its minimum-image centre distance exceeds list radius + both
-  // radii.  The sequential reduction of md_min_image_fixed is the minimum
-  // image in a rectangular box; in a reduced triclinic one the minimum is one
-  // of the 27 images around it, so their smallest distance is used.  A box
-  // that needs image enumeration keeps every SJ.  The j-clusters then go
-  // through the same tests in the same order as a scan of every J >= S's
-  // first cluster, so the list is the one that scan builds.
+  // skipped when no image of its centre comes within list radius + both
+  // radii.  Without enumeration that image is the minimum image, which the
+  // sequential reduction of md_min_image_fixed gives in a rectangular box; in
+  // a reduced triclinic one it is one of the 27 images around the reduced
+  // difference.  With enumeration, the images within Kp shells of it, Kp from
+  // the same bound as K with the largest super-cluster radius in place of the
+  // largest cluster radius.  The j-clusters then go through the same tests in
+  // the same order as a scan of every J >= S's first cluster, so the list is
+  // the one that scan builds.
   __shared__ int cand[MD_LIST_CANDIDATES];
   int ncand = 0;
-  if (!enumerate) {
-    const float rs = V.rs;
+  const float rs = V.rs;
+  int Kp[3] = {c.box.rectangular ? 0 : 1, c.box.rectangular ? 0 : 1, c.box.rectangular ? 0 : 1};
+  bool prefilter = true;
+  if (enumerate) {
+    const float Rp = c.list_cutoff + rs + __uint_as_float(ctl->super_radius_bits);
+    for (int d = 0; d < 3; ++d) {
+      const int k = max(1, (int)ceilf(Rp / c.width[d] + c.fmax[d] - 1.0f));
+      prefilter &= k <= MD_MAX_IMAGE_SHELLS;
+      Kp[d] = min(k, MD_MAX_IMAGE_SHELLS);
+    }
+  }
+  if (prefilter) {
     for (int sb = S; sb < c.nsuper; sb += 64) {
       const int SJ = sb + lane;
       bool near = false;
@@ -547,10 +558,9 @@ __global__ void __launch_bounds__(64) k_nb_list(const md_nb_args *__restrict__ T
           int64_t dx = L.super_origin[SJ] - ox, dy = L.super_origin[c.nsuper + SJ] - oy,
                   dz = L.super_origin[2 * c.nsuper + SJ] - oz;
           md_min_image_fixed(dx, dy, dz, c.box);
-          const int k = c.box.rectangular ? 0 : 1;
-          for (int kc = -k; kc <= k; ++kc)
-            for (int kb = -k; kb <= k; ++kb)
-              for (int ka = -k; ka <= k; ++ka) {
+          for (int kc = -Kp[2]; kc <= Kp[2]; ++kc)
+            for (int kb = -Kp[1]; kb <= Kp[1]; ++kb)
+              for (int ka = -Kp[0]; ka <= Kp[0]; ++ka) {
                 float fx = md_from_fixed(dx + ka * c.box.fa[0] + kb * c.box.fb[0] + kc * c.box.fc[0]);
                 float fy = md_from_fixed(dy + kb * c.box.fb[1] + kc * c.box.fc[1]);
                 float fz = md_from_fixed(dz + kc * c.box.fc[2]);
@@ -565,8 +575,8 @@ __global__ void __launch_bounds__(64) k_nb_list(const md_nb_args *__restrict__ T
     }
     __syncthreads();
   }
-  // past the candidate store, or enumerating images: scan every J
-  const bool scan_all = enumerate || ncand > MD_LIST_CANDIDATES;
+  // past the candidate store, or images too far out to prefilter: scan every J
+  const bool scan_all = !prefilter || ncand > MD_LIST_CANDIDATES;
   const int chunks = scan_all ? (c.nclusters - first + 63) / 64 : (ncand + 7) / 8;
   int written = 0;
   int base_out = PASS == 1 ? L.super_count[S] : 0;
@@ -695,6 +705,7 @@ __global__ void __launch_bounds__(64) k_nb_masks(const md_nb_args *__restrict__ 
   for (int e = eb + part; e < ee; e += MD_MASK_WAVES) {
     const int J = L.entry_j[e];
     const int meta = L.entry_imask[e];
+    if (lane == 0) atomicAdd(&L.j_count[J], 1);
     const int ja = slot_atom[J * MD_CLUSTER + jl];
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
@@ -755,9 +766,12 @@ __global__ void k_nb_pack(const md_nb_args *__restrict__ T, md_bmap map, int *se
   if (bad) A.ctl->overflow = 3;
 }
 
-// After the pair kernel: each slot's force onto its atom, and the slot cleared
-// for the next step.  Kernels on one stream run in order and a slot is the
-// only one holding its atom, so a plain read-add-write is enough.
+// After the pair kernel: each slot's force onto its atom -- its i-force sums
+// plus, as j atom, the force of every list entry naming its cluster, each
+// rounded to Q32.32 on its own (the value an atomic per entry would add) --
+// and the slot cleared for the next step.  Kernels on one stream run in order
+// and a slot is the only one holding its atom, so a plain read-add-write is
+// enough.
 __global__ void k_nb_forces(const md_nb_args *__restrict__ T, md_bmap map) {
   int blk;
   const md_nb_args &A = md_nb_entry(T, map, blk);
@@ -766,10 +780,43 @@ __global__ void k_nb_forces(const md_nb_args *__restrict__ T, md_bmap map) {
   if (s >= nslots) return;
   int64_t *slot_force = A.P.slot_force;
   const int a = A.slot_atom[s];
+  int64_t f[3];
   for (int d = 0; d < 3; ++d) {
-    const int64_t v = slot_force[d * nslots + s];
+    f[d] = slot_force[d * nslots + s];
     slot_force[d * nslots + s] = 0;
-    if (a >= 0) A.force[d * np + a] += v;
+  }
+  if (a < 0) return;
+  const int J = s / MD_CLUSTER, q = s - J * MD_CLUSTER;
+  const float *jf = A.P.j_force;
+  for (int k = A.L.j_start[J], end = A.L.j_start[J + 1]; k < end; ++k) {
+    const float *e = jf + (size_t)A.L.j_entries[k] * 24 + q;
+    f[0] += md_to_fixed(e[0]);
+    f[1] += md_to_fixed(e[8]);
+    f[2] += md_to_fixed(e[16]);
+  }
+  for (int d = 0; d < 3; ++d) A.force[d * np + a] += f[d];
+}
+
+// The transposed list: starts from the counts k_nb_masks took (one
+// workgroup per replica), then each entry's number placed under its J.
+__global__ void __launch_bounds__(1024) k_nb_j_scan(const md_nb_args *__restrict__ T, md_bmap map) {
+  int blk;
+  const md_nb_args &A = md_nb_entry(T, map, blk);
+  if (!A.ctl->rebuild_active) return;
+  md_block_scan(A.L.j_count, A.L.j_start, A.c.nclusters, A.L.j_count);
+}
+
+__global__ void __launch_bounds__(64) k_nb_j_fill(const md_nb_args *__restrict__ T, md_bmap map) {
+  int blk;
+  const md_nb_args &A = md_nb_entry(T, map, blk);
+  if (!A.ctl->rebuild_active) return;
+  const md_list_ptrs &L = A.L;
+  const int S = blk / MD_MASK_WAVES, part = blk % MD_MASK_WAVES;
+  if (S >= A.c.nsuper) return;
+  const int eb = L.super_count[S], ee = min(L.super_count[S + 1], A.totals[0]);
+  for (int e = eb + part * 64 + (int)threadIdx.x; e < ee; e += MD_MASK_WAVES * 64) {
+    const int J = L.entry_j[e];
+    L.j_entries[L.j_start[J] + atomicAdd(&L.j_count[J], 1)] = e;
   }
 }
 
@@ -841,13 +888,12 @@ __device__ __forceinline__ void md_pair_segment(const md_pair_args &A, int seg, 
   int64_t ax = 0, ay = 0, az = 0;
   int unit_end = min(eb + MD_SEGMENT, ee);
   for (int e = eb; e < ee; ++e) {
-    int J = A.entry_j[e];
-    int meta = A.entry_imask[e];
+    const int J = A.entry_j[e];
+    const int meta = A.entry_imask[e];
     if (ENERGY) blocks += __popc(meta & 0xff);
-    int4 sh = A.entry_shift[e];
-    int jslot = J * MD_CLUSTER + jl;
-    int4 pj = A.slot_xyzq[jslot];
-    float2 lj = A.slot_lj[jslot];
+    const int4 sh = A.entry_shift[e];
+    const int4 pj = A.slot_xyzq[J * MD_CLUSTER + jl];
+    const float2 lj = A.slot_lj[J * MD_CLUSTER + jl];
     const float qj = __int_as_float(pj.w);
     // wrapping (unsigned) integer arithmetic: exact for every real pair; the
     // padding slots far out may wrap, and their pairs are masked
@@ -882,12 +928,8 @@ __device__ __forceinline__ void md_pair_segment(const md_pair_args &A, int seg, 
       gy += __shfl_xor(gy, off, 64);
       gz += __shfl_xor(gz, off, 64);
     }
-    if (lane < 8) {  // padding slots collect zeros and are never moved to an atom
-      const int js = J * MD_CLUSTER + lane;
-      md_atomic_add_i64(&A.slot_force[js], md_to_fixed(gx));
-      md_atomic_add_i64(&A.slot_force[A.nslots + js], md_to_fixed(gy));
-      md_atomic_add_i64(&A.slot_force[2 * A.nslots + js], md_to_fixed(gz));
-    }
+    // lanes 0-23 hold (x, y, z) of j atoms 0-7: one 96-byte store
+    if (lane < 24) A.j_force[(size_t)e * 24 + lane] = lane < 8 ? gx : (lane < 16 ? gy : gz);
     if (e + 1 == unit_end) {  // wave-uniform: the end of a unit
       // i forces: sum the 8 lanes that share i (same il), fixed order; every
       // lane of the butterfly ends with the same sum, lane jl keeps q = jl's
@@ -1052,6 +1094,8 @@ struct md_nb_buffers {
   uint64_t *entry_excl = nullptr;
   uint64_t *excl_window = nullptr;
   int64_t *slot_force = nullptr;
+  int *j_count = nullptr, *j_start = nullptr, *j_entries = nullptr;
+  float *j_force = nullptr;
   int *totals = nullptr;
   int nsuper = 0;
   int entry_capacity = 0, segment_capacity = 0;
@@ -1145,6 +1189,10 @@ void md_nb_setup(md_system *s) {
   B.entry_imask = md_alloc<int>(s, B.entry_capacity);
   B.entry_shift = md_alloc<int4>(s, B.entry_capacity);
   B.entry_excl = md_alloc<uint64_t>(s, (size_t)B.entry_capacity * 8);
+  B.j_count = md_alloc<int>(s, nb.nclusters);
+  B.j_start = md_alloc<int>(s, nb.nclusters + 1);
+  B.j_entries = md_alloc<int>(s, B.entry_capacity);
+  B.j_force = md_alloc<float>(s, (size_t)B.entry_capacity * 24);
   B.segments = md_alloc<int4>(s, B.segment_capacity);
   s->excl_start_d = md_alloc<int>(s, s->n + 1);
   s->excl_list_d = md_alloc<int>(s, std::max<size_t>(1, s->excl_list.size()));
@@ -1232,12 +1280,16 @@ static md_nb_args nb_args(md_system *s, int seg_len) {
   L.entry_shift = B.entry_shift;
   L.segments = B.segments;
   L.entry_excl = B.entry_excl;
+  L.j_count = B.j_count;
+  L.j_start = B.j_start;
+  L.j_entries = B.j_entries;
   md_pair_args &P = a.P;
   P.segments = B.segments;
   P.entry_j = B.entry_j;
   P.entry_imask = B.entry_imask;
   P.entry_shift = B.entry_shift;
   P.entry_excl = B.entry_excl;
+  P.j_force = B.j_force;
   P.slot_xyzq = s->slot_xyzq;
   P.slot_lj = s->slot_lj;
   P.totals = B.totals;
@@ -1359,11 +1411,15 @@ void md_nb_build(md_group *g) {
   k_nb_scatter<<<G.atoms.total, 256, 0, st>>>(T, G.atoms.map);
   k_nb_rank<<<G.atoms.total, 256, 0, st>>>(T, G.atoms.map);
   k_nb_clusters<<<G.clusters.total, 64, 0, st>>>(T, G.clusters.map);
+  md_mark(g, MD_PHASE_SEARCH);
   k_nb_list<0><<<G.supers.total, 64, 0, st>>>(T, G.supers.map);
   k_nb_list_scan<<<G.scan.total, 1024, 0, st>>>(T, G.scan.map);
   k_nb_list<1><<<G.supers.total, 64, 0, st>>>(T, G.supers.map);
   k_nb_segments<<<G.segments.total, 64, 0, st>>>(T, G.segments.map);
+  md_mark(g, MD_PHASE_MASKS);
   k_nb_masks<<<G.masks.total, 64, 0, st>>>(T, G.masks.map);
+  k_nb_j_scan<<<G.scan.total, 1024, 0, st>>>(T, G.scan.map);
+  k_nb_j_fill<<<G.masks.total, 64, 0, st>>>(T, G.masks.map);
   MD_HIP(hipGetLastError());
 }
 
@@ -1384,6 +1440,7 @@ void md_nb_pairs(md_group *g, bool energy) {
     else
       k_nb_pairs<false, false><<<G.pair_grid[0], 64, 0, st>>>(G.table, G.seg.map, G.cursor);
   }
+  md_mark(g, MD_PHASE_FORCES);
   k_nb_forces<<<G.slots.total, 256, 0, st>>>(G.table, G.slots.map);
   MD_HIP(hipGetLastError());
 }

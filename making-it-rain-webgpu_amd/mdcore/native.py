@@ -390,7 +390,7 @@ class NativeBatch:
         return {library().md_phase_name(k).decode(): float(ms[k]) for k in range(PHASE_COUNT)}
 
 
-PHASE_COUNT = 9
+PHASE_COUNT = 12
 
 
 def probe_fixed_point(values):

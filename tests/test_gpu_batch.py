@@ -95,7 +95,7 @@ def test_batch_profile_and_lifetime(replicas):
     batch = HipBatch(engines)
     ph = batch.profileStep(3)
     print(ph)
-    for name in ("pairs", "bonded", "pme_spread", "pme_fft_convolve", "pme_gather", "integrate"):
+    for name in ("pairs", "pair_forces", "bonded", "pme_spread", "pme_fft_convolve", "pme_gather", "integrate"):
         assert ph[name] > 0.0
     batch.destroy()
     engines[0].run(5)  # a replica goes on alone after its batch

@@ -139,7 +139,7 @@ def test_profile_reports_every_phase(prepared):
     e = _engine(b, pos, vel, step=step)
     ph = e.profileStep(5)
     print(ph)
-    for name in ("pairs", "bonded", "pme_spread", "pme_fft_convolve", "pme_gather", "integrate"):
+    for name in ("pairs", "pair_forces", "bonded", "pme_spread", "pme_fft_convolve", "pme_gather", "integrate"):
         assert ph[name] > 0.0
     assert e.steps == step + 5
     e.destroy()
