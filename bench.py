@@ -49,6 +49,11 @@ import numpy as np  # noqa: E402
 FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
 PAIR_FLOPS = 76            # algorithmic FP32 flops per in-cutoff pair
+# HBM bytes per launch of the batched pair kernel on this ensemble: FETCH_SIZE +
+# WRITE_SIZE from two rocprofv3 --pmc passes over `bench.py --traffic-probe 3`,
+# each scaled by its reading for a 256 MiB device copy
+# (profiles/k_nb_pairs_ensemble64_traffic.md)
+PAIR_TRAFFIC_BYTES = 734.5e6
 DT_PS = 0.002
 METRIC = "aggregate ns/day at 2 fs (batched+learned vs plain, same GPU, fixed force error)"
 
@@ -141,7 +146,7 @@ def constraint_error(engine, system):
     return float(np.max(np.abs(d - system.constraintDistance) / system.constraintDistance))
 
 
-def pair_roofline(pairs, kernel_ms):
+def pair_roofline(pairs, kernel_ms, traffic=None, traffic_source=None):
     achieved = pairs * PAIR_FLOPS / (kernel_ms * 1e-3) / 1e12 if kernel_ms > 0 else 0.0
     return {
         "kernel": "k_nb_pairs (LJ + erfc-Ewald cluster-pair kernel)",
@@ -150,7 +155,8 @@ def pair_roofline(pairs, kernel_ms):
         "peak": FP32_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": achieved / FP32_PEAK_TFLOPS,
-        "traffic": None,
+        "traffic": traffic,
+        "traffic_source": traffic_source,
         "algorithmic": f"{PAIR_FLOPS} flop x {int(pairs)} in-cutoff pairs per launch",
         "kernel_ms": kernel_ms,
     }
@@ -336,7 +342,9 @@ def run_ensemble(args):
             "arithmetic": "fp32 with Q32.32 int64 accumulation",
             "parallelism": "single GPU",
         },
-        "roofline": pair_roofline(pairs, phases["pairs"]),
+        "roofline": pair_roofline(pairs, phases["pairs"], PAIR_TRAFFIC_BYTES,
+                                  "profiles/k_nb_pairs_ensemble64_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                                  "scaled by a 256 MiB copy)"),
         "phases_ms": phases,
         "integrator_roofline": {"bound": "hbm", "achieved": integ_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": integ_gbs / HBM_PEAK_GBS, "bytes_per_atom": 124},
