@@ -68,6 +68,7 @@ struct md_build_ctx {
   int nclusters, nsuper, nslots;
   int entry_capacity, segment_capacity;
   int seg_len;  // list entries per pair-kernel work unit (MD_SEGMENT or a multiple)
+  int entries_per_super;  // the entry region of each super-cluster
   float list_cutoff2;
   float half_min_width;
   float width[3];  // perpendicular widths along a, b, c
@@ -89,7 +90,7 @@ struct md_list_ptrs {
   const float4 *cluster_half;
   const int64_t *super_origin;
   const float *super_radius;  // half diagonal of each super-cluster's bounding box (nm)
-  int *super_count;   // [nsuper+1] entries per super-cluster, then starts
+  int *super_count;   // [nsuper] entries per super-cluster (in its region)
   int *super_segs;    // [nsuper+1] segments per super-cluster, then starts
   int *entry_j;
   int *entry_imask;
@@ -469,10 +470,11 @@ __device__ __forceinline__ void md_candidate(const md_build_ctx &c, const md_lis
   });
 }
 
-// Count (PASS 0) or write (PASS 1) the list entries of one super-cluster: one
-// wavefront per S, lanes over candidate J clusters, compacted with a
-// wavefront prefix count so the order is deterministic.
-template <int PASS>
+// Write the list entries of one super-cluster into its own region of the
+// entry arrays (entries_per_super entries from S * entries_per_super) and
+// their count: one wavefront per S, lanes over candidate J clusters,
+// compacted with a wavefront prefix count so the order is deterministic.  A
+// count past the region reports a list overflow.
 __global__ void __launch_bounds__(64) k_nb_list(const md_nb_args *__restrict__ T, md_bmap map) {
   int S;
   const md_nb_args &A = md_nb_entry(T, map, S);
@@ -520,7 +522,7 @@ __global__ void __launch_bounds__(64) k_nb_list(const md_nb_args *__restrict__ T
       too_far |= k > MD_MAX_IMAGE_SHELLS;
       K[d] = min(k, MD_MAX_IMAGE_SHELLS);
     }
-  if (PASS == 0 && lane == 0 && too_far) ctl->overflow = 2;
+  if (lane == 0 && too_far) ctl->overflow = 2;
   // Candidate j super-clusters SJ >= S, ascending.  A j-cluster of SJ can
   // reach an i-cluster of S only if the bounding spheres of the two
   // super-clusters come within the list radius at some image, so SJ is
@@ -579,7 +581,7 @@ __global__ void __launch_bounds__(64) k_nb_list(const md_nb_args *__restrict__ T
   const bool scan_all = !prefilter || ncand > MD_LIST_CANDIDATES;
   const int chunks = scan_all ? (c.nclusters - first + 63) / 64 : (ncand + 7) / 8;
   int written = 0;
-  int base_out = PASS == 1 ? L.super_count[S] : 0;
+  const int base_out = S * c.entries_per_super, region_end = base_out + c.entries_per_super;
   for (int chunk = 0; chunk < chunks; ++chunk) {
     int J;
     if (scan_all) {
@@ -606,12 +608,12 @@ __global__ void __launch_bounds__(64) k_nb_list(const md_nb_args *__restrict__ T
       int v = __shfl_up(incl, off, 64);
       if (lane >= off) incl += v;
     }
-    if (PASS == 1 && valid && count) {
+    if (valid && count) {
       int at = base_out + written + incl - count;
       int SJ = J / 8;
       int64_t jox = L.super_origin[SJ], joy = L.super_origin[c.nsuper + SJ], joz = L.super_origin[2 * c.nsuper + SJ];
       auto write = [&](int meta, int64_t shx, int64_t shy, int64_t shz) {
-        if (at < c.entry_capacity) {
+        if (at < region_end) {
           L.entry_j[at] = J;
           L.entry_imask[at] = meta;
           // J's frame origin, moved into this image, in S's frame
@@ -631,7 +633,10 @@ __global__ void __launch_bounds__(64) k_nb_list(const md_nb_args *__restrict__ T
     }
     written += __shfl(incl, 63, 64);
   }
-  if (PASS == 0 && lane == 0) L.super_count[S] = written;
+  if (lane == 0) {
+    L.super_count[S] = written;
+    if (written > c.entries_per_super) ctl->overflow = 1;
+  }
 }
 
 // Starts of every super-cluster's entries and segments, and the totals.
@@ -642,18 +647,25 @@ __global__ void __launch_bounds__(1024) k_nb_list_scan(const md_nb_args *__restr
   md_control *ctl = A.ctl;
   if (!ctl->rebuild_active) return;
   const md_list_ptrs &L = A.L;
-  const int nsuper = A.c.nsuper, entry_capacity = A.c.entry_capacity, segment_capacity = A.c.segment_capacity;
+  const int nsuper = A.c.nsuper, region = A.c.entries_per_super, segment_capacity = A.c.segment_capacity;
   int *totals = A.totals;
-  for (int s = threadIdx.x; s < nsuper; s += 1024) L.super_segs[s] = (L.super_count[s] + A.c.seg_len - 1) / A.c.seg_len;
+  __shared__ int entries;
+  if (threadIdx.x == 0) entries = 0;
   __syncthreads();
-  md_block_scan(L.super_count, L.super_count, nsuper, nullptr);
+  int mine = 0;
+  for (int s = threadIdx.x; s < nsuper; s += 1024) {
+    const int n = min(L.super_count[s], region);
+    L.super_segs[s] = (n + A.c.seg_len - 1) / A.c.seg_len;
+    mine += n;
+  }
+  atomicAdd(&entries, mine);
   __syncthreads();
   md_block_scan(L.super_segs, L.super_segs, nsuper, nullptr);
   __syncthreads();
   if (threadIdx.x == 0) {
-    int entries = L.super_count[nsuper], segs = L.super_segs[nsuper];
-    if (entries > entry_capacity || segs > segment_capacity) ctl->overflow = 1;
-    totals[0] = min(entries, entry_capacity);
+    int segs = L.super_segs[nsuper];
+    if (segs > segment_capacity) ctl->overflow = 1;
+    totals[0] = entries;
     totals[1] = min(segs, segment_capacity);
   }
 }
@@ -663,15 +675,15 @@ __global__ void k_nb_segments(const md_nb_args *__restrict__ T, md_bmap map) {
   const md_nb_args &A = md_nb_entry(T, map, blk);
   if (!A.ctl->rebuild_active) return;
   const md_list_ptrs &L = A.L;
-  const int entry_capacity = A.c.entry_capacity, segment_capacity = A.c.segment_capacity;
+  const int segment_capacity = A.c.segment_capacity, region = A.c.entries_per_super;
   int S = blk * blockDim.x + threadIdx.x;
   if (S >= A.c.nsuper) return;
-  int b = L.super_count[S], e = L.super_count[S + 1];
+  const int b = S * region, e = b + min(L.super_count[S], region);
   int sg = L.super_segs[S];
   const int len = A.c.seg_len;
   for (int at = b; at < e; at += len, ++sg) {
     if (sg >= segment_capacity) break;
-    L.segments[sg] = make_int4(S, min(at, entry_capacity), min(min(at + len, e), entry_capacity), 0);
+    L.segments[sg] = make_int4(S, at, min(at + len, e), 0);
   }
 }
 
@@ -691,7 +703,6 @@ __global__ void __launch_bounds__(64) k_nb_masks(const md_nb_args *__restrict__ 
   const int S = blk / MD_MASK_WAVES, part = blk % MD_MASK_WAVES;
   if (S >= c.nsuper) return;
   const int *slot_atom = A.slot_atom, *excl_start = A.excl_start, *excl_list = A.excl_list;
-  const int total = A.totals[0];
   const int lane = threadIdx.x, il = lane >> 3, jl = lane & 7;
   int ia[8];
   uint64_t iw[8];
@@ -701,7 +712,7 @@ __global__ void __launch_bounds__(64) k_nb_masks(const md_nb_args *__restrict__ 
     ia[q] = I < c.nclusters ? slot_atom[I * MD_CLUSTER + il] : -1;
     iw[q] = ia[q] >= 0 ? A.excl_window[ia[q]] : 0;
   }
-  const int eb = L.super_count[S], ee = min(L.super_count[S + 1], total);
+  const int eb = S * c.entries_per_super, ee = eb + min(L.super_count[S], c.entries_per_super);
   for (int e = eb + part; e < ee; e += MD_MASK_WAVES) {
     const int J = L.entry_j[e];
     const int meta = L.entry_imask[e];
@@ -813,7 +824,7 @@ __global__ void __launch_bounds__(64) k_nb_j_fill(const md_nb_args *__restrict__
   const md_list_ptrs &L = A.L;
   const int S = blk / MD_MASK_WAVES, part = blk % MD_MASK_WAVES;
   if (S >= A.c.nsuper) return;
-  const int eb = L.super_count[S], ee = min(L.super_count[S + 1], A.totals[0]);
+  const int eb = S * A.c.entries_per_super, ee = eb + min(L.super_count[S], A.c.entries_per_super);
   for (int e = eb + part * 64 + (int)threadIdx.x; e < ee; e += MD_MASK_WAVES * 64) {
     const int J = L.entry_j[e];
     L.j_entries[L.j_start[J] + atomicAdd(&L.j_count[J], 1)] = e;
@@ -1098,7 +1109,7 @@ struct md_nb_buffers {
   float *j_force = nullptr;
   int *totals = nullptr;
   int nsuper = 0;
-  int entry_capacity = 0, segment_capacity = 0;
+  int entry_capacity = 0, segment_capacity = 0, entries_per_super = 0;
   float half_min_width = 0, list_cutoff = 0;
 };
 
@@ -1168,7 +1179,7 @@ void md_nb_setup(md_system *s) {
   B.cluster_center = md_alloc<int64_t>(s, 3 * (size_t)nb.nclusters);
   B.super_origin = md_alloc<int64_t>(s, 3 * (size_t)B.nsuper);
   B.super_radius = md_alloc<float>(s, B.nsuper);
-  B.super_count = md_alloc<int>(s, B.nsuper + 1);
+  B.super_count = md_alloc<int>(s, B.nsuper);
   B.super_segs = md_alloc<int>(s, B.nsuper + 1);
   B.totals = md_alloc<int>(s, 4);
   // list capacity: the cluster pairs within the list radius plus the cluster
@@ -1182,9 +1193,12 @@ void md_nb_setup(md_system *s) {
   double side = std::cbrt(MD_SUPER / std::max(density, 1.0));
   double span = side + 2.0 * (rlist + ext);
   double per_super = span * span * span * density / MD_CLUSTER;
-  long cap = (long)(1.5 * B.nsuper * std::min(per_super, 729.0 * nb.nclusters)) + 1024;
-  B.entry_capacity = (int)std::min(cap, (long)INT32_MAX / 16);
-  B.segment_capacity = B.entry_capacity / MD_SEGMENT + B.nsuper + 64;
+  // per super-cluster region, a multiple of the shortest segment
+  long region = (long)(1.5 * std::min(per_super, 729.0 * nb.nclusters)) + 64;
+  region = std::min((region + MD_SEGMENT - 1) / MD_SEGMENT * MD_SEGMENT, (long)INT32_MAX / 16 / B.nsuper);
+  B.entries_per_super = (int)region;
+  B.entry_capacity = (int)(region * B.nsuper);
+  B.segment_capacity = B.nsuper * (int)(region / MD_SEGMENT) + 64;
   B.entry_j = md_alloc<int>(s, B.entry_capacity);
   B.entry_imask = md_alloc<int>(s, B.entry_capacity);
   B.entry_shift = md_alloc<int4>(s, B.entry_capacity);
@@ -1235,6 +1249,7 @@ static md_build_ctx build_ctx(md_system *s) {
   c.nsuper = B.nsuper;
   c.nslots = s->nb.nslots;
   c.entry_capacity = B.entry_capacity;
+  c.entries_per_super = B.entries_per_super;
   c.segment_capacity = B.segment_capacity;
   c.list_cutoff2 = s->nb.list_cutoff2;
   c.list_cutoff = B.list_cutoff;
@@ -1412,9 +1427,8 @@ void md_nb_build(md_group *g) {
   k_nb_rank<<<G.atoms.total, 256, 0, st>>>(T, G.atoms.map);
   k_nb_clusters<<<G.clusters.total, 64, 0, st>>>(T, G.clusters.map);
   md_mark(g, MD_PHASE_SEARCH);
-  k_nb_list<0><<<G.supers.total, 64, 0, st>>>(T, G.supers.map);
+  k_nb_list<<<G.supers.total, 64, 0, st>>>(T, G.supers.map);
   k_nb_list_scan<<<G.scan.total, 1024, 0, st>>>(T, G.scan.map);
-  k_nb_list<1><<<G.supers.total, 64, 0, st>>>(T, G.supers.map);
   k_nb_segments<<<G.segments.total, 64, 0, st>>>(T, G.segments.map);
   md_mark(g, MD_PHASE_MASKS);
   k_nb_masks<<<G.masks.total, 64, 0, st>>>(T, G.masks.map);
