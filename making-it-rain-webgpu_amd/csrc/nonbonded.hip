@@ -29,6 +29,7 @@
 //     is converted to Q32.32 before it is added anywhere, so two identical
 //     runs are bit-identical whatever order the hardware schedules waves in.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <vector>
 
@@ -119,11 +120,15 @@ struct md_pair_args {
   // per list entry, the f32 force on each of J's 8 atoms, [entry][x|y|z][8]:
   // one plain store per entry instead of 24 atomics; k_nb_forces sums them
   float *j_force;
+  // each cluster's current bounding box in its frame (frame units; a cluster
+  // of padding only holds an inverted box), written by k_nb_pack
+  const int4 *cluster_lo, *cluster_hi;
   int64_t *energy;
   int64_t *census;  // in-cutoff pairs (md_control)
   int64_t *blocks;  // 8x8 cluster-pair blocks evaluated (md_control)
   int nslots, nclusters;
   float cutoff2, alpha, beta, exp_scale;
+  float prune2;  // cutoff^2 (1 + 1e-5): a block whose boxes are this far apart has no pair within the cutoff
 };
 
 // Everything the nonbonded kernels of one replica read: its table entry.
@@ -144,6 +149,7 @@ struct md_nb_args {
   const int *excl_start, *excl_list;
   const uint64_t *excl_window;
   int64_t *force;  // the atoms' Q32.32 forces, [3][np]
+  int4 *cluster_lo, *cluster_hi;
   int *totals;
   md_control *ctl;
 };
@@ -761,20 +767,38 @@ __global__ void k_nb_pack(const md_nb_args *__restrict__ T, md_bmap map, int *se
   const int64_t *pos = A.pos, *slot_offset = A.slot_offset;
   int4 *slot_xyzq = A.slot_xyzq;
   int s = blk * blockDim.x + threadIdx.x;
-  if (s >= nslots) return;
-  int a = slot_atom[s];
+  if (s >= nslots) return;  // whole wavefronts: nslots is a multiple of 64
+  const int a = slot_atom[s];
+  int lo[3], hi[3];
   if (a < 0) {
     // far away (20 nm out) and uncharged; every pair with it is masked anyway
     const int far = 20 << 26;
     slot_xyzq[s] = make_int4(far + ((s & 63) << 20), far, far, 0);
-    return;
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = INT_MAX;
+      hi[d] = INT_MIN;
+    }
+  } else {
+    bool bad = false;
+    int x = md_frame_coord(pos[a] - slot_offset[s], bad);
+    int y = md_frame_coord(pos[np + a] - slot_offset[nslots + s], bad);
+    int z = md_frame_coord(pos[2 * np + a] - slot_offset[2 * nslots + s], bad);
+    slot_xyzq[s] = make_int4(x, y, z, __float_as_int(A.atom_param[a].x));
+    if (bad) A.ctl->overflow = 3;
+    lo[0] = hi[0] = x;
+    lo[1] = hi[1] = y;
+    lo[2] = hi[2] = z;
   }
-  bool bad = false;
-  int x = md_frame_coord(pos[a] - slot_offset[s], bad);
-  int y = md_frame_coord(pos[np + a] - slot_offset[nslots + s], bad);
-  int z = md_frame_coord(pos[2 * np + a] - slot_offset[2 * nslots + s], bad);
-  slot_xyzq[s] = make_int4(x, y, z, __float_as_int(A.atom_param[a].x));
-  if (bad) A.ctl->overflow = 3;
+  // the cluster's current box over its atoms: 8 consecutive lanes
+  for (int d = 0; d < 3; ++d)
+    for (int off = 1; off < MD_CLUSTER; off <<= 1) {
+      lo[d] = min(lo[d], __shfl_xor(lo[d], off, 64));
+      hi[d] = max(hi[d], __shfl_xor(hi[d], off, 64));
+    }
+  if ((s & (MD_CLUSTER - 1)) == 0) {
+    A.cluster_lo[s / MD_CLUSTER] = make_int4(lo[0], lo[1], lo[2], 0);
+    A.cluster_hi[s / MD_CLUSTER] = make_int4(hi[0], hi[1], hi[2], 0);
+  }
 }
 
 // After the pair kernel: each slot's force onto its atom -- its i-force sums
@@ -865,6 +889,13 @@ __device__ __forceinline__ float md_pair(float r2, float qi, float si, float ei,
   return fs;
 }
 
+// gap between boxes [ilo, ihi] and [jlo, jhi] on one axis, in nm (frame
+// units; exact integer differences for boxes within 32 nm of each other)
+__device__ __forceinline__ float md_box_gap(int ilo, int ihi, int jlo, int jhi) {
+  const int a = (int)((unsigned)jlo - (unsigned)ihi), b = (int)((unsigned)ilo - (unsigned)jhi);
+  return (float)max(0, max(a, b)) * MD_FRAME_UNIT;
+}
+
 // One segment: consecutive list entries of super-cluster S against their j
 // clusters, in units of MD_SEGMENT entries counted from S's first entry.  The
 // i forces and the energy are summed in f32 over one unit and rounded to
@@ -898,46 +929,72 @@ __device__ __forceinline__ void md_pair_segment(const md_pair_args &A, int seg, 
   // lane (il, jl) keeps the i force of atom il of S's cluster q = jl
   int64_t ax = 0, ay = 0, az = 0;
   int unit_end = min(eb + MD_SEGMENT, ee);
+  // lanes 0-7: the current box of S's cluster q = lane
+  int4 ilo = make_int4(0, 0, 0, 0), ihi = ilo;
+  if (lane < 8) {
+    ilo = A.cluster_lo[S * 8 + lane];
+    ihi = A.cluster_hi[S * 8 + lane];
+  }
   for (int e = eb; e < ee; ++e) {
     const int J = A.entry_j[e];
     const int meta = A.entry_imask[e];
-    if (ENERGY) blocks += __popc(meta & 0xff);
     const int4 sh = A.entry_shift[e];
-    const int4 pj = A.slot_xyzq[J * MD_CLUSTER + jl];
-    const float2 lj = A.slot_lj[J * MD_CLUSTER + jl];
-    const float qj = __int_as_float(pj.w);
-    // wrapping (unsigned) integer arithmetic: exact for every real pair; the
-    // padding slots far out may wrap, and their pairs are masked
-    unsigned xj = (unsigned)pj.x + (unsigned)sh.x, yj = (unsigned)pj.y + (unsigned)sh.y,
-             zj = (unsigned)pj.z + (unsigned)sh.z;
-    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
-    const uint64_t *ex = A.entry_excl + (size_t)e * 8;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      if ((meta >> q) & 1) {
-        uint64_t m = ex[q];
-        bool allowed = (m >> lane) & 1;
-        float dx = (float)(int)(xj - (unsigned)xi[q]), dy = (float)(int)(yj - (unsigned)yi[q]),
-              dz = (float)(int)(zj - (unsigned)zi[q]);
-        float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz)) * MD_FRAME_UNIT2;
-        // a block whose 64 pairs all lie beyond the cutoff (or are masked)
-        // adds exactly zero: skip its arithmetic, wave-uniformly
-        if (__ballot(allowed && r2 < A.cutoff2) == 0) continue;
-        float fs = md_pair<ENERGY>(r2, qi[q], si[q], ei[q], qj, lj.x, lj.y, allowed, A, en, count);
-        fx[q] = fmaf(-fs, dx, fx[q]);
-        fy[q] = fmaf(-fs, dy, fy[q]);
-        fz[q] = fmaf(-fs, dz, fz[q]);
-        gx = fmaf(fs, dx, gx);
-        gy = fmaf(fs, dy, gy);
-        gz = fmaf(fs, dz, gz);
-      }
+    // Blocks whose current boxes lie more than the cutoff apart (by a 1e-5
+    // relative margin over the f32 distance test) have no pair inside it and
+    // would add exactly zero: drop them before any pair arithmetic.  Lane q
+    // tests S's cluster q; integer box differences wrap exactly for boxes
+    // within 32 nm, and a box of padding only (inverted) may test either way,
+    // its pairs being masked.
+    int active = 0;
+    {
+      const int4 jlo = A.cluster_lo[J], jhi = A.cluster_hi[J];
+      const float dx = md_box_gap(ilo.x, ihi.x, (int)((unsigned)jlo.x + (unsigned)sh.x),
+                                  (int)((unsigned)jhi.x + (unsigned)sh.x));
+      const float dy = md_box_gap(ilo.y, ihi.y, (int)((unsigned)jlo.y + (unsigned)sh.y),
+                                  (int)((unsigned)jhi.y + (unsigned)sh.y));
+      const float dz = md_box_gap(ilo.z, ihi.z, (int)((unsigned)jlo.z + (unsigned)sh.z),
+                                  (int)((unsigned)jhi.z + (unsigned)sh.z));
+      const bool near = lane < 8 && fmaf(dx, dx, fmaf(dy, dy, dz * dz)) < A.prune2;
+      active = meta & (int)(__ballot(near) & 0xffu);
     }
-    // j force: sum the 8 lanes that share j (lane%8), fixed order
+    if (ENERGY) blocks += __popc(active);
+    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+    if (active) {
+      const int4 pj = A.slot_xyzq[J * MD_CLUSTER + jl];
+      const float2 lj = A.slot_lj[J * MD_CLUSTER + jl];
+      const float qj = __int_as_float(pj.w);
+      // wrapping (unsigned) integer arithmetic: exact for every real pair; the
+      // padding slots far out may wrap, and their pairs are masked
+      unsigned xj = (unsigned)pj.x + (unsigned)sh.x, yj = (unsigned)pj.y + (unsigned)sh.y,
+               zj = (unsigned)pj.z + (unsigned)sh.z;
+      const uint64_t *ex = A.entry_excl + (size_t)e * 8;
 #pragma unroll
-    for (int off = 8; off < 64; off <<= 1) {
-      gx += __shfl_xor(gx, off, 64);
-      gy += __shfl_xor(gy, off, 64);
-      gz += __shfl_xor(gz, off, 64);
+      for (int q = 0; q < 8; ++q) {
+        if ((active >> q) & 1) {
+          uint64_t m = ex[q];
+          bool allowed = (m >> lane) & 1;
+          float dx = (float)(int)(xj - (unsigned)xi[q]), dy = (float)(int)(yj - (unsigned)yi[q]),
+                dz = (float)(int)(zj - (unsigned)zi[q]);
+          float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz)) * MD_FRAME_UNIT2;
+          // a block whose 64 pairs all lie beyond the cutoff (or are masked)
+          // adds exactly zero: skip its arithmetic, wave-uniformly
+          if (__ballot(allowed && r2 < A.cutoff2) == 0) continue;
+          float fs = md_pair<ENERGY>(r2, qi[q], si[q], ei[q], qj, lj.x, lj.y, allowed, A, en, count);
+          fx[q] = fmaf(-fs, dx, fx[q]);
+          fy[q] = fmaf(-fs, dy, fy[q]);
+          fz[q] = fmaf(-fs, dz, fz[q]);
+          gx = fmaf(fs, dx, gx);
+          gy = fmaf(fs, dy, gy);
+          gz = fmaf(fs, dz, gz);
+        }
+      }
+      // j force: sum the 8 lanes that share j (lane%8), fixed order
+#pragma unroll
+      for (int off = 8; off < 64; off <<= 1) {
+        gx += __shfl_xor(gx, off, 64);
+        gy += __shfl_xor(gy, off, 64);
+        gz += __shfl_xor(gz, off, 64);
+      }
     }
     // lanes 0-23 hold (x, y, z) of j atoms 0-7: one 96-byte store
     if (lane < 24) A.j_force[(size_t)e * 24 + lane] = lane < 8 ? gx : (lane < 16 ? gy : gz);
@@ -1107,6 +1164,7 @@ struct md_nb_buffers {
   int64_t *slot_force = nullptr;
   int *j_count = nullptr, *j_start = nullptr, *j_entries = nullptr;
   float *j_force = nullptr;
+  int4 *cluster_lo = nullptr, *cluster_hi = nullptr;
   int *totals = nullptr;
   int nsuper = 0;
   int entry_capacity = 0, segment_capacity = 0, entries_per_super = 0;
@@ -1207,6 +1265,8 @@ void md_nb_setup(md_system *s) {
   B.j_start = md_alloc<int>(s, nb.nclusters + 1);
   B.j_entries = md_alloc<int>(s, B.entry_capacity);
   B.j_force = md_alloc<float>(s, (size_t)B.entry_capacity * 24);
+  B.cluster_lo = md_alloc<int4>(s, nb.nslots / MD_CLUSTER);
+  B.cluster_hi = md_alloc<int4>(s, nb.nslots / MD_CLUSTER);
   B.segments = md_alloc<int4>(s, B.segment_capacity);
   s->excl_start_d = md_alloc<int>(s, s->n + 1);
   s->excl_list_d = md_alloc<int>(s, std::max<size_t>(1, s->excl_list.size()));
@@ -1305,6 +1365,8 @@ static md_nb_args nb_args(md_system *s, int seg_len) {
   P.entry_shift = B.entry_shift;
   P.entry_excl = B.entry_excl;
   P.j_force = B.j_force;
+  P.cluster_lo = B.cluster_lo;
+  P.cluster_hi = B.cluster_hi;
   P.slot_xyzq = s->slot_xyzq;
   P.slot_lj = s->slot_lj;
   P.totals = B.totals;
@@ -1315,6 +1377,7 @@ static md_nb_args nb_args(md_system *s, int seg_len) {
   P.nslots = s->nb.nslots;
   P.nclusters = s->nb.nclusters;
   P.cutoff2 = s->nb.cutoff2;
+  P.prune2 = (float)((double)s->nb.cutoff2 * (1.0 + 1e-5));
   P.alpha = (float)s->alpha;
   P.beta = (float)(2.0 * s->alpha / std::sqrt(M_PI));
   P.exp_scale = (float)(-s->alpha * s->alpha / std::log(2.0));
@@ -1341,6 +1404,8 @@ static md_nb_args nb_args(md_system *s, int seg_len) {
   a.excl_list = s->excl_list_d;
   a.excl_window = B.excl_window;
   a.force = s->force;
+  a.cluster_lo = B.cluster_lo;
+  a.cluster_hi = B.cluster_hi;
   a.totals = B.totals;
   a.ctl = s->ctl;
   return a;
