@@ -67,7 +67,7 @@ __device__ __forceinline__ float md_erfcx_poly(float t) {
 struct md_build_ctx {
   int n, np, ncells, gdim0, gdim1, gdim2;
   int nclusters, nsuper, nslots;
-  int entry_capacity, segment_capacity;
+  int segment_capacity;
   int seg_len;  // list entries per pair-kernel work unit (MD_SEGMENT or a multiple)
   int entries_per_super;  // the entry region of each super-cluster
   float list_cutoff2;
@@ -1308,7 +1308,6 @@ static md_build_ctx build_ctx(md_system *s) {
   c.nclusters = s->nb.nclusters;
   c.nsuper = B.nsuper;
   c.nslots = s->nb.nslots;
-  c.entry_capacity = B.entry_capacity;
   c.entries_per_super = B.entries_per_super;
   c.segment_capacity = B.segment_capacity;
   c.list_cutoff2 = s->nb.list_cutoff2;
