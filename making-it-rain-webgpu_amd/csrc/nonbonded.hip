@@ -83,8 +83,11 @@ struct md_build_ctx {
 #define MD_MAX_IMAGE_SHELLS 4
 // candidate j super-clusters a list wave keeps in LDS (beyond: scan all)
 #define MD_LIST_CANDIDATES 1024
-// wavefronts that share one super-cluster's entries in the mask kernel
+// wavefronts that share one super-cluster's entries in the mask kernel, and
+// wavefronts per workgroup (a replica that did not rebuild dispatches
+// MD_MASK_WAVES / MD_MASK_GROUP workgroups per super-cluster that return at once)
 #define MD_MASK_WAVES 16
+#define MD_MASK_GROUP 4
 
 struct md_list_ptrs {
   const int64_t *cluster_center;
@@ -700,16 +703,17 @@ __global__ void k_nb_segments(const md_nb_args *__restrict__ T, md_bmap map) {
 // excluded pair (the CSR holds both directions; nonbonded.ts:159-182).  An
 // exclusion is looked up in the i atom's 64-bit index window, and in the CSR
 // only for the rare atom flagged as having exclusions outside it.
-__global__ void __launch_bounds__(64) k_nb_masks(const md_nb_args *__restrict__ T, md_bmap map) {
+__global__ void __launch_bounds__(64 * MD_MASK_GROUP) k_nb_masks(const md_nb_args *__restrict__ T, md_bmap map) {
   int blk;
   const md_nb_args &A = md_nb_entry(T, map, blk);
   if (!A.ctl->rebuild_active) return;
   const md_build_ctx &c = A.c;
   const md_list_ptrs &L = A.L;
-  const int S = blk / MD_MASK_WAVES, part = blk % MD_MASK_WAVES;
+  const int wave = blk * MD_MASK_GROUP + (int)(threadIdx.x >> 6);
+  const int S = wave / MD_MASK_WAVES, part = wave % MD_MASK_WAVES;
   if (S >= c.nsuper) return;
   const int *slot_atom = A.slot_atom, *excl_start = A.excl_start, *excl_list = A.excl_list;
-  const int lane = threadIdx.x, il = lane >> 3, jl = lane & 7;
+  const int lane = threadIdx.x & 63, il = lane >> 3, jl = lane & 7;
   int ia[8];
   uint64_t iw[8];
 #pragma unroll
@@ -841,15 +845,16 @@ __global__ void __launch_bounds__(1024) k_nb_j_scan(const md_nb_args *__restrict
   md_block_scan(A.L.j_count, A.L.j_start, A.c.nclusters, A.L.j_count);
 }
 
-__global__ void __launch_bounds__(64) k_nb_j_fill(const md_nb_args *__restrict__ T, md_bmap map) {
+__global__ void __launch_bounds__(64 * MD_MASK_GROUP) k_nb_j_fill(const md_nb_args *__restrict__ T, md_bmap map) {
   int blk;
   const md_nb_args &A = md_nb_entry(T, map, blk);
   if (!A.ctl->rebuild_active) return;
   const md_list_ptrs &L = A.L;
-  const int S = blk / MD_MASK_WAVES, part = blk % MD_MASK_WAVES;
+  const int wave = blk * MD_MASK_GROUP + (int)(threadIdx.x >> 6);
+  const int S = wave / MD_MASK_WAVES, part = wave % MD_MASK_WAVES;
   if (S >= A.c.nsuper) return;
   const int eb = S * A.c.entries_per_super, ee = eb + min(L.super_count[S], A.c.entries_per_super);
-  for (int e = eb + part * 64 + (int)threadIdx.x; e < ee; e += MD_MASK_WAVES * 64) {
+  for (int e = eb + part * 64 + (int)(threadIdx.x & 63); e < ee; e += MD_MASK_WAVES * 64) {
     const int J = L.entry_j[e];
     L.j_entries[L.j_start[J] + atomicAdd(&L.j_count[J], 1)] = e;
   }
@@ -1434,7 +1439,7 @@ void md_nb_group_setup(md_group *g) {
     clusters.push_back((s->nb.nclusters + 63) / 64);
     supers.push_back(B.nsuper);
     segments.push_back((B.nsuper + 63) / 64);
-    masks.push_back(B.nsuper * MD_MASK_WAVES);
+    masks.push_back(B.nsuper * (MD_MASK_WAVES / MD_MASK_GROUP));
     slots.push_back((s->nb.nslots + 255) / 256);
   }
   G->atoms = md_make_launch(g, atoms);
@@ -1495,9 +1500,9 @@ void md_nb_build(md_group *g) {
   k_nb_list_scan<<<G.scan.total, 1024, 0, st>>>(T, G.scan.map);
   k_nb_segments<<<G.segments.total, 64, 0, st>>>(T, G.segments.map);
   md_mark(g, MD_PHASE_MASKS);
-  k_nb_masks<<<G.masks.total, 64, 0, st>>>(T, G.masks.map);
+  k_nb_masks<<<G.masks.total, 64 * MD_MASK_GROUP, 0, st>>>(T, G.masks.map);
   k_nb_j_scan<<<G.scan.total, 1024, 0, st>>>(T, G.scan.map);
-  k_nb_j_fill<<<G.masks.total, 64, 0, st>>>(T, G.masks.map);
+  k_nb_j_fill<<<G.masks.total, 64 * MD_MASK_GROUP, 0, st>>>(T, G.masks.map);
   MD_HIP(hipGetLastError());
 }
 
