@@ -196,6 +196,8 @@ def run_ensemble(args):
         # blow up within tens of steps; the reference protocol equilibrates
         # between minimisation and production for the same reason
         opts = dict(temperature=r.temperature, constraintTolerance=1e-6, seed=r.seed, maxBatch=args.max_batch)
+        if args.verlet_skin is not None:
+            opts["verletSkin"] = args.verlet_skin
         g = HipEngine(r.built.system, r.built.state,
                       RunOptions(timeStep=args.equilibrate_dt, friction=args.equilibrate_friction, **opts))
         res = minimise(g, args.minimise_iterations, 10.0)
@@ -281,6 +283,7 @@ def run_ensemble(args):
         f"{warm_rate:.3f} SHAKE passes per cluster-step")
     deviations = [constraint_error(e, r.built.system) for e, r in zip(engines, reps)]
     worst_constraint = max(deviations)
+    diverged = sum(1 for d in deviations if not d < 1e-4)
     log(f"worst relative constraint deviation after the learned run: {worst_constraint:.2e}")
     if worst_constraint > 1e-4:
         bad = [(atoms[k], f"{d:.1e}") for k, d in enumerate(deviations) if d > 1e-4]
@@ -336,6 +339,7 @@ def run_ensemble(args):
                                    "loss_first": loss0, "loss_last": loss1, "seconds": train_s,
                                    "network": "32-64-64-16 tanh MLP, bf16 MFMA 16x16x16, fp32 accumulation"},
             "worst_constraint_rel_deviation": worst_constraint,
+            "diverged_replicas": diverged,
             "steps_per_list_rebuild": steps_per_rebuild,
             "force_rel_rms_vs_oracle": errors,
             "in_cutoff_pairs": int(pairs),
@@ -482,6 +486,8 @@ def main():
     ap.add_argument("--learning-rate", type=float, default=1e-3)
     ap.add_argument("--ws-capacity", type=int, default=4_000_000)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--verlet-skin", type=float, default=None,
+                    help="list skin in nm (default: the engine's, 0.15)")
     ap.add_argument("--max-batch", type=int, default=0,
                     help="steps per captured HIP graph (0: 50; 1: launch every step directly)")
     ap.add_argument("--ensemble-cache", default="",
