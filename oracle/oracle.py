@@ -287,10 +287,59 @@ class OracleSystem:
     def free_split(self, sp):
         lib().orc_split_free(C.byref(sp))
 
+    def iterative_split(self, constraints):
+        """A split that sends the given constraint indices to SHAKE/RATTLE and
+        none to SETTLE (not freed with ``free_split``: numpy owns it)."""
+        idx = np.ascontiguousarray(constraints, dtype=np.int32)
+        self._keep.append(idx)
+        sp = _Split()
+        sp.n_iterative = int(idx.shape[0])
+        sp.iterative = _i(idx)
+        return sp
+
+    def settle_positions(self, sp, reference, trial):
+        """SETTLE every rigid water of ``trial`` (in place) against ``reference``."""
+        lib().orc_settle_positions(self.ref, C.byref(sp), _d(reference), _d(trial))
+
+    def settle_velocities(self, sp, positions, velocities):
+        lib().orc_settle_velocities(self.ref, C.byref(sp), _d(positions), _d(velocities))
+
+    def shake(self, sp, reference, trial, tolerance, max_iterations=150):
+        """SHAKE the iterative clusters of ``trial`` in place: (iterations, converged)."""
+        conv = C.c_int(0)
+        it = lib().orc_shake(self.ref, C.byref(sp), _d(reference), _d(trial), tolerance,
+                             max_iterations, C.byref(conv))
+        return int(it), bool(conv.value)
+
+    def rattle(self, sp, positions, velocities, tolerance, max_iterations=150):
+        conv = C.c_int(0)
+        it = lib().orc_rattle(self.ref, C.byref(sp), _d(positions), _d(velocities), tolerance,
+                              max_iterations, C.byref(conv))
+        return int(it), bool(conv.value)
+
     def lfmiddle_step(self, sp, positions, velocities, forces, dt, temperature, friction,
                       tolerance, seed, step):
         lib().orc_lfmiddle_step(self.ref, C.byref(sp), _d(positions), _d(velocities), _d(forces),
                                 dt, temperature, friction, tolerance, seed, step)
+
+    def verlet_step(self, sp, positions, velocities, forces, dt, tolerance):
+        lib().orc_verlet_step(self.ref, C.byref(sp), _d(positions), _d(velocities), _d(forces), dt,
+                              tolerance)
+
+    def velocities_at_temperature(self, sp, positions, temperature, seed, forces=None, offset=0.0):
+        v = np.zeros(3 * self.c.n)
+        f = None if forces is None else _d(np.ascontiguousarray(forces, dtype=np.float64))
+        lib().orc_velocities_at_temperature(self.ref, C.byref(sp),
+                                            _d(np.ascontiguousarray(positions, dtype=np.float64)), f,
+                                            temperature, seed, offset, _d(v))
+        return v
+
+    def position_restraint(self, positions, box):
+        """(energy, forces) of the PositionRestraint terms alone."""
+        f = np.zeros(3 * self.c.n)
+        e = lib().orc_position_restraint(self.ref, _d(np.ascontiguousarray(positions, dtype=np.float64)),
+                                         _d(_box9(box)), _d(f))
+        return e, f
 
     def remove_com(self, velocities):
         lib().orc_remove_com(self.ref, _d(velocities))
