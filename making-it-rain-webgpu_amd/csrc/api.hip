@@ -116,8 +116,10 @@ static void group_release(md_group *g) {
   if (!g) return;
   hipSetDevice(g->device);
   if (g->stream) hipStreamSynchronize(g->stream);
-  if (g->stride_exec) hipGraphExecDestroy(g->stride_exec);
-  if (g->stride_graph) hipGraphDestroy(g->stride_graph);
+  for (auto &c : g->graphs) {
+    if (c.exec) hipGraphExecDestroy(c.exec);
+    if (c.graph) hipGraphDestroy(c.graph);
+  }
   if (g->nb) md_nb_group_free(g);
   if (g->bonded) md_bonded_group_free(g);
   if (g->pme) md_pme_group_free(g);
@@ -153,6 +155,7 @@ static md_group *group_create(const std::vector<md_system *> &reps, hipStream_t 
     }
     if (any_nb && !all_nb)
       throw md_error(MD_ERR_INVALID_ARGUMENT, "a batch mixes systems with and without a NonbondedForce");
+    g->status = md_group_alloc<uint32_t>(g, 1);
     if (any_nb) {
       md_nb_group_setup(g);
       md_pme_group_setup(g);
@@ -597,17 +600,26 @@ static void launch_step(md_group *g) {
 static void check_control(md_system *s) {
   md_control c;
   MD_HIP(hipMemcpy(&c, s->ctl, sizeof c, hipMemcpyDeviceToHost));
+  const std::string when = " (at or before step " + std::to_string(c.step_current) + ")";
   if (c.overflow == 2)
     throw md_error(MD_ERR_CUTOFF_TOO_LARGE,
                    "the list radius plus the cluster extents reaches past four periodic images along an axis: "
-                   "the box is too small for this cutoff");
+                   "the box is too small for this cutoff" + when);
   if (c.overflow == 3)
     throw md_error(MD_ERR_FIXED_POINT_OVERFLOW,
-                   "an atom moved more than 32 nm from its cluster's frame: the simulation has diverged");
+                   "an atom moved more than 32 nm from its cluster's frame: the simulation has diverged" + when);
   if (c.overflow)
     throw md_error(MD_ERR_LIST_OVERFLOW,
                    "the cluster-pair list outgrew its capacity (the system compressed far beyond its starting "
-                   "density); recreate the system from the current state");
+                   "density); set positions at a lower density or recreate the system from the current state" +
+                       when);
+  if (c.diverged)
+    throw md_error(MD_ERR_CONSTRAINTS_DIVERGED,
+                   "the constraints diverged" + when +
+                       ((c.diverged & 1) ? ": SHAKE did not converge within its passes" : "") +
+                       ((c.diverged & 2) ? ": RATTLE did not converge within its passes" : "") +
+                       ((c.diverged & 4) ? ": a velocity became non-finite" : "") +
+                       "; the time step is too long for this system, or its starting geometry is strained");
 }
 
 // host state of each replica onto the device before its first step: the
@@ -636,35 +648,70 @@ static void fault_tick(md_system *s, long count) {
   s->fault_after -= count;
 }
 
+static void drop_graphs(md_group *g) {
+  for (auto &c : g->graphs) {
+    if (c.exec) MD_HIP(hipGraphExecDestroy(c.exec));
+    if (c.graph) MD_HIP(hipGraphDestroy(c.graph));
+    c = md_group::captured{};
+  }
+}
+
+// the captured graph of `steps` steps: slot 0 holds whole strides, slot 1 the
+// last shorter run (captured again when a run of another length comes)
+static hipGraphExec_t stride_graph(md_group *g, int steps, int stride) {
+  md_group::captured &c = g->graphs[steps == stride ? 0 : 1];
+  if (c.exec && c.steps == steps) return c.exec;
+  if (c.exec) MD_HIP(hipGraphExecDestroy(c.exec));
+  if (c.graph) MD_HIP(hipGraphDestroy(c.graph));
+  c = md_group::captured{};
+  MD_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+  try {
+    for (int k = 0; k < steps; ++k) launch_step(g);
+  } catch (...) {
+    hipGraph_t broken = nullptr;
+    (void)hipStreamEndCapture(g->stream, &broken);
+    if (broken) (void)hipGraphDestroy(broken);
+    throw;
+  }
+  MD_HIP(hipStreamEndCapture(g->stream, &c.graph));
+  MD_HIP(hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0));
+  c.steps = steps;
+  return c.exec;
+}
+
+// Advance every replica by `steps`, one stride (a captured graph) at a time.
+// After each stride the group's status word -- set by a step's prologue when
+// any replica's control block holds an error -- is read, so a run whose list
+// overflows or whose constraints diverge stops within one stride of it.  A
+// run that ends in an error leaves every replica's step count unchanged.
 static void run_group(md_group *g, int64_t steps) {
   if (steps <= 0) return;
   MD_HIP(hipSetDevice(g->device));
   group_prepare(g);
   md_zero_forces(g, false);
-  int64_t left = steps;
   const int stride = g->max_batch > 0 ? g->max_batch : STRIDE_STEPS;
-  if (g->graph_stale && g->stride_exec) {
-    MD_HIP(hipGraphExecDestroy(g->stride_exec));
-    MD_HIP(hipGraphDestroy(g->stride_graph));
-    g->stride_exec = nullptr;
-    g->stride_graph = nullptr;
-  }
+  if (g->graph_stale) drop_graphs(g);
   g->graph_stale = false;
-  if (stride > 1 && left >= stride && !g->stride_exec) {
-    MD_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < stride; ++k) launch_step(g);
-    MD_HIP(hipStreamEndCapture(g->stream, &g->stride_graph));
-    MD_HIP(hipGraphInstantiate(&g->stride_exec, g->stride_graph, nullptr, nullptr, 0));
-    g->stride_steps = stride;
+  MD_HIP(hipMemsetAsync(g->status, 0, sizeof(uint32_t), g->stream));
+  int64_t done = 0;
+  while (done < steps) {
+    const int chunk = (int)std::min<int64_t>(stride, steps - done);
+    if (chunk > 1)
+      MD_HIP(hipGraphLaunch(stride_graph(g, chunk, stride), g->stream));
+    else
+      launch_step(g);
+    uint32_t status = 0;
+    MD_HIP(hipMemcpyAsync(&status, g->status, sizeof status, hipMemcpyDeviceToHost, g->stream));
+    MD_HIP(hipStreamSynchronize(g->stream));
+    if (status) {
+      for (md_system *s : g->reps) check_control(s);
+      throw md_error(MD_ERR_HIP, "a replica reported an error that its control block no longer holds");
+    }
+    done += chunk;
   }
-  while (g->stride_exec && left >= g->stride_steps) {
-    MD_HIP(hipGraphLaunch(g->stride_exec, g->stream));
-    left -= g->stride_steps;
-  }
-  for (; left > 0; --left) launch_step(g);
-  MD_HIP(hipStreamSynchronize(g->stream));
-  for (md_system *s : g->reps) s->steps_done += steps;
+  // an error raised in the last step is seen by no later prologue
   for (md_system *s : g->reps) check_control(s);
+  for (md_system *s : g->reps) s->steps_done += steps;
 }
 
 static void run_steps(md_system *s, int64_t steps) {
@@ -822,6 +869,10 @@ md_err md_set_positions(md_system *s, const double *x) {
   MD_HIP(hipMemcpyAsync(s->pos, f.data(), sizeof(int64_t) * f.size(), hipMemcpyHostToDevice, s->stream));
   // setPositions clears the carried residual (src/gpu/loop.ts:819-833)
   MD_HIP(hipMemsetAsync(s->resid, 0, sizeof(float) * 3 * s->np, s->stream));
+  // new positions start a new trajectory: the next step rebuilds the list
+  // and raises any overflow again
+  MD_HIP(hipMemsetAsync(&s->ctl->overflow, 0, sizeof(uint32_t), s->stream));
+  MD_HIP(hipMemsetAsync(&s->ctl->diverged, 0, sizeof(uint32_t), s->stream));
   MD_HIP(hipStreamSynchronize(s->stream));
   s->need_rebuild = true;
   MD_API_END
@@ -833,7 +884,13 @@ md_err md_set_velocities(md_system *s, const double *v) {
   MD_HIP(hipSetDevice(s->device));
   std::vector<float> f(3 * (size_t)s->np, 0.0f);
   for (int i = 0; i < s->n; ++i)
-    for (int d = 0; d < 3; ++d) f[(size_t)d * s->np + i] = (float)v[3 * i + d];
+    for (int d = 0; d < 3; ++d) {
+      float x = (float)v[3 * i + d];
+      if (!std::isfinite(x))
+        throw md_error(MD_ERR_INVALID_ARGUMENT, "velocity " + std::to_string(v[3 * i + d]) + " of particle " +
+                                                    std::to_string(i) + " is not a finite f32 value");
+      f[(size_t)d * s->np + i] = x;
+    }
   MD_HIP(hipMemcpy(s->vel, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice));
   s->momentum_dirty = true;
   MD_API_END

@@ -45,7 +45,8 @@ struct md_control {
   int64_t momentum[2][3];          // sum m v in Q32.32, by step parity
   int64_t pair_census;             // in-cutoff pairs, when counted
   uint32_t force_pending;          // the host asks for a rebuild at the next step
-  uint32_t pad_;
+  uint32_t diverged;               // the integrator's constraint solve failed: bit 0 SHAKE did not
+                                   // converge, bit 1 RATTLE did not, bit 2 a non-finite velocity
   int64_t shake_passes;            // SHAKE update passes over all cluster-steps
   int64_t shake_clusters;          // cluster-steps SHAKE has run
   int64_t pair_blocks;             // 8x8 cluster-pair blocks evaluated, when counted
@@ -105,11 +106,18 @@ struct md_group {
   int max_cluster_atoms = 0;
   // per-subsystem launch tables (each file's private struct)
   void *nb = nullptr, *bonded = nullptr, *pme = nullptr, *integ = nullptr;
-  // captured stride
-  hipGraphExec_t stride_exec = nullptr;
-  hipGraph_t stride_graph = nullptr;
-  int stride_steps = 0;
+  // captured strides: a whole stride and the last shorter remainder run
+  struct captured {
+    int steps = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+  };
+  captured graphs[2];
   int max_batch = 0;  // steps per captured graph (0: the default)
+  // set by each step's prologue when a replica's control block holds an
+  // error (list or frame overflow, diverged constraints); read after every
+  // stride so a failing run stops within one stride
+  uint32_t *status = nullptr;
   int ws_mode = 0;          // cluster integrator: 0 plain, 1 recording, 2 learned
   bool graph_stale = false; // the captured stride no longer matches the kernels
   // profiling: events recorded by md_mark while `profiling` is set

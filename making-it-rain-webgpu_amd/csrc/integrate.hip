@@ -33,6 +33,7 @@ struct md_int_args {
   const float *mass;
   const int64_t *build_pos;
   md_control *ctl;
+  uint32_t *group_status;  // the launch group's error word (see k_step_prologue)
   int np;
   float dt, half_dt, inv_dt;
   float vscale, noise_scale, kt;
@@ -153,7 +154,7 @@ __device__ void md_drift(const md_int_args &A, md_item<MAXA> &it, uint32_t step)
 
 // velocity from the constraint correction, commit, momentum, displacement
 template <int MAXA>
-__device__ void md_finish(const md_int_args &A, md_item<MAXA> &it, double *mom, float &worst) {
+__device__ void md_finish(const md_int_args &A, md_item<MAXA> &it, double *mom, float &worst, uint32_t &failed) {
   for (int a = 0; a < it.n; ++a) {
     int i = it.atom[a];
     if (it.im[a] != 0.0f) {
@@ -169,6 +170,7 @@ __device__ void md_finish(const md_int_args &A, md_item<MAXA> &it, double *mom, 
     float d2 = 0.0f;
     for (int d = 0; d < 3; ++d) {
       A.vel[d * A.np + i] = it.v[a][d];
+      if (!isfinite(it.v[a][d])) failed |= 4u;
       mom[d] += (double)m * it.v[a][d];
       float disp = md_from_fixed(A.pos[d * A.np + i] - A.build_pos[d * A.np + i]);
       d2 += disp * disp;
@@ -183,8 +185,9 @@ __device__ __forceinline__ double md_fx(int64_t v) { return (double)v * 2.328306
 
 // RATTLE (src/reference/constraints.ts:195-243): Jacobi, along the current
 // bond vectors, divided by the reference squared length
+// (false: not converged within MD_MAX_CONSTRAINT_ITERATIONS passes)
 template <int MAXA>
-__device__ void md_rattle(const md_int_args &A, md_item<MAXA> &it, int nc, const int4 *cons) {
+__device__ bool md_rattle(const md_int_args &A, md_item<MAXA> &it, int nc, const int4 *cons) {
   double rv[MD_MAX_CLUSTER_CONSTRAINTS][3], l2[MD_MAX_CLUSTER_CONSTRAINTS], mu[MD_MAX_CLUSTER_CONSTRAINTS];
   double v[MAXA][3];
   for (int a = 0; a < it.n; ++a)
@@ -200,6 +203,7 @@ __device__ void md_rattle(const md_int_args &A, md_item<MAXA> &it, int nc, const
     mu[c] = sum == 0.0 ? 0.0 : 0.5 / sum;
   }
   double delta[MD_MAX_CLUSTER_CONSTRAINTS];
+  bool converged = false;
   for (int iter = 0; iter < MD_MAX_CONSTRAINT_ITERATIONS; ++iter) {
     int conv = 0;
     for (int c = 0; c < nc; ++c) {
@@ -209,7 +213,10 @@ __device__ void md_rattle(const md_int_args &A, md_item<MAXA> &it, int nc, const
       delta[c] = l2[c] == 0.0 ? 0.0 : -2.0 * mu[c] * proj / l2[c];
       conv += fabs(delta[c]) <= A.tol;
     }
-    if (conv == nc) break;
+    if (conv == nc) {
+      converged = true;
+      break;
+    }
     for (int c = 0; c < nc; ++c) {
       int li = cons[c].x, lj = cons[c].y;
       for (int d = 0; d < 3; ++d) {
@@ -221,10 +228,13 @@ __device__ void md_rattle(const md_int_args &A, md_item<MAXA> &it, int nc, const
   }
   for (int a = 0; a < it.n; ++a)
     for (int d = 0; d < 3; ++d) it.v[a][d] = (float)v[a][d];
+  return converged || nc == 0;
 }
 
 // SHAKE (src/reference/constraints.ts:130-186): Jacobi along the reference
-// bond vectors, converged when |r'|^2 lies in [(1-t)^2, (1+t)^2] d^2
+// bond vectors, converged when |r'|^2 lies in [(1-t)^2, (1+t)^2] d^2.
+// *passes receives the update passes applied, or -1 when the cluster did not
+// converge within MD_MAX_CONSTRAINT_ITERATIONS.
 template <int MAXA>
 __device__ void md_shake(const md_int_args &A, md_item<MAXA> &it, int nc, const int4 *cons,
                          const float *lam0 = nullptr, double *lam_total = nullptr, int *passes = nullptr) {
@@ -292,6 +302,7 @@ __device__ void md_shake(const md_int_args &A, md_item<MAXA> &it, int nc, const 
   }
   double total[MD_MAX_CLUSTER_CONSTRAINTS];
   for (int c = 0; c < nc; ++c) total[c] = delta[c];
+  bool converged = nc == 0;
   for (int iter = 0; iter < MD_MAX_CONSTRAINT_ITERATIONS; ++iter) {
     int conv = 0;
     for (int c = 0; c < nc; ++c) {
@@ -302,7 +313,10 @@ __device__ void md_shake(const md_int_args &A, md_item<MAXA> &it, int nc, const 
       delta[c] = proj == 0.0 ? 0.0 : mu[c] * (t2[c] - l2) / proj;
       conv += (l2 >= lower * t2[c] && l2 <= upper * t2[c]);
     }
-    if (conv == nc) break;
+    if (conv == nc) {
+      converged = true;
+      break;
+    }
     ++applied;
     for (int c = 0; c < nc; ++c) {
       int li = cons[c].x, lj = cons[c].y;
@@ -318,7 +332,7 @@ __device__ void md_shake(const md_int_args &A, md_item<MAXA> &it, int nc, const 
   }
   for (int a = 0; a < it.n; ++a)
     for (int d = 0; d < 3; ++d) it.tr[a][d] += (int64_t)rint(moved[a][d] * 4294967296.0);
-  if (passes) *passes = applied;
+  if (passes) *passes = converged ? applied : -1;
   if (lam_total)
     for (int c = 0; c < nc && c < MD_WS_MAXC; ++c) lam_total[c] = mu[c] == 0.0 ? 0.0 : total[c] / mu[c];
 }
@@ -533,6 +547,7 @@ __global__ void __launch_bounds__(256) k_integrate(const md_int_args *__restrict
   md_item<MAXA> it;
   it.n = 0;
   int nc = 0;
+  uint32_t failed = 0;  // control-block divergence bits this thread raises
   int4 cons[KIND == 2 ? MD_MAX_CLUSTER_CONSTRAINTS : 1];
   if (live) {
     if (KIND == 0) {
@@ -560,7 +575,7 @@ __global__ void __launch_bounds__(256) k_integrate(const md_int_args *__restrict
       float2 dd = A.water_dist[t];
       md_settle_positions(it, dd.x, dd.y, m3);
     } else if constexpr (KIND == 2) {
-      if (A.integrator == 0) md_rattle(A, it, nc, cons);
+      if (A.integrator == 0 && !md_rattle(A, it, nc, cons)) failed |= 2u;
       md_drift(A, it, step);
     } else {
       md_drift(A, it, step);
@@ -592,7 +607,11 @@ __global__ void __launch_bounds__(256) k_integrate(const md_int_args *__restrict
       float f[MD_WS_F];
       if (MODE == 1 && covered) md_ws_features(it, nc, cons, f);
       md_shake(A, it, nc, cons, warm ? lam0 : nullptr, MODE == 1 ? total : nullptr, &passes);
-      if (MODE == 1 && covered && passes < MD_MAX_CONSTRAINT_ITERATIONS) {
+      if (passes < 0) {
+        failed |= 1u;
+        passes = MD_MAX_CONSTRAINT_ITERATIONS;
+      }
+      if (MODE == 1 && covered && !(failed & 1u)) {
         // a cluster-step SHAKE did not converge teaches nothing: its slot
         // keeps the zero features (no constraint present) it was cleared to
         int64_t k = (A.ctl->step_current - A.rec_first_step) * (int64_t)A.n_ccl + t;
@@ -610,7 +629,8 @@ __global__ void __launch_bounds__(256) k_integrate(const md_int_args *__restrict
       md_atomic_add_i64(&A.ctl->shake_clusters, cs);
     }
   }
-  if (live) md_finish(A, it, mom, worst);
+  if (live) md_finish(A, it, mom, worst, failed);
+  if (failed) atomicOr(&A.ctl->diverged, failed);
   md_reduce_out(A, step, mom, worst);
 }
 
@@ -678,11 +698,13 @@ __global__ void __launch_bounds__(256) k_constrain(const md_int_args *__restrict
 
 // start of a step, one thread per replica: advance the device step counter,
 // latch the rebuild request (the displacement test's, or the host's),
-// clear the momentum slot this step's integrator fills
+// clear the momentum slot this step's integrator fills, and raise the group's
+// status word when an earlier step left an error in the control block
 __global__ void k_step_prologue(const md_int_args *__restrict__ T, int R, int advance) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= R) return;
   md_control *ctl = T[r].ctl;
+  if (ctl->overflow | ctl->diverged) atomicOr(T[r].group_status, 1u);
   if (advance) {
     ctl->step_current = ctl->step_next;
     ctl->step_next = ctl->step_next + 1;
@@ -795,6 +817,7 @@ void md_int_group_refresh(md_group *g) {
   int mode = 0;
   for (md_system *s : g->reps) {
     host.push_back(int_args(s));
+    host.back().group_status = g->status;
     if (s->rec_cap > 0) mode = 1;
     else if (s->ws_net && mode == 0) mode = 2;
   }

@@ -56,12 +56,20 @@ class FaultInjected(MdError):
     pass
 
 
+class ConstraintsDiverged(MdError):
+    pass
+
+
+class ListOverflow(MdError):
+    pass
+
+
 class NoDevice(MdError):
     pass
 
 
-_SPECIFIC = {2: CutoffTooLargeForBox, 3: ClusterTooLarge, 4: FixedPointOverflow, 11: FaultInjected,
-             12: NoDevice}
+_SPECIFIC = {2: CutoffTooLargeForBox, 3: ClusterTooLarge, 4: FixedPointOverflow, 8: ListOverflow,
+             10: ConstraintsDiverged, 11: FaultInjected, 12: NoDevice}
 
 _ip = C.POINTER(C.c_int32)
 _dp = C.POINTER(C.c_double)

@@ -56,8 +56,12 @@ def test_one_step_matches_oracle_given_device_forces(prepared):
     o.lfmiddle_step(sp, x, v, forces, 0.002, 300.0, 1.0, 1e-6, 5, step)
     o.free_split(sp)
     err = np.max(np.abs(x_dev - x))
-    print("one-step max position difference (nm):", err, "velocity:", np.max(np.abs(v_dev - v)))
+    verr = np.max(np.abs(v_dev - v))
+    print("one-step max position difference (nm):", err, "velocity (nm/ps):", verr)
     assert err < 2e-7
+    # the velocity carries the constraint correction (x' - x_unconstrained)/dt,
+    # so the position gate over dt bounds it
+    assert verr < 2e-7 / 0.002
     e.destroy()
 
 
@@ -154,4 +158,85 @@ def test_fault_injection_surfaces_as_an_error(prepared):
         e.run(10)
     e._native.inject_fault(-1)
     e.run(2)
+    e.destroy()
+
+
+def test_stale_list_evaluates_every_in_cutoff_pair(prepared):
+    """The gate upstream's neighbour validation sets (validate-neighbours.ts:
+    1-12): a list that misses a pair gives forces a force comparison cannot
+    tell apart, and a trajectory that blows up later.  Step until the list is
+    as stale as the rebuild rule allows (an atom has moved more than a
+    quarter of the skin since the build, and no rebuild has happened in the
+    last step), then count on the device the in-cutoff pairs the pair kernel
+    evaluates from that stale list: it must equal the oracle's count at the
+    same positions, pairs within f32 rounding of the cutoff excepted, and the
+    count a freshly built list gives."""
+    from scipy.spatial import cKDTree
+    b, pos, vel, step = prepared
+    e = _engine(b, pos, vel, step=step, seed=13)
+    e.run(1)
+    skin = 0.15
+    stale = None
+    for k in range(60):
+        before = e._native.list_flags()
+        e.run(1)
+        after = e._native.list_flags()
+        # between a quarter and half the skin: no rebuild is pending either
+        if after["rebuilds"] == before["rebuilds"] and 0.25 * skin < after["worst_displacement"] < 0.5 * skin:
+            stale = after
+            break
+    assert stale is not None, "the list never became stale enough"
+    pairs_stale, _ = e._native.pair_census()
+    assert e._native.list_flags()["rebuilds"] == stale["rebuilds"]  # the census did not rebuild
+    x = e.positions()
+    fresh = _engine(b, x, e.velocities(), step=e.steps)
+    pairs_fresh, _ = fresh._native.pair_census()
+    o = O.OracleSystem(b.system)
+    d, _ = o.direct_space(x, b.system.boxVectors, with_forces=False)
+    L = np.diag(np.asarray(b.system.boxVectors))
+    tree = cKDTree(np.mod(x.reshape(-1, 3), L), boxsize=L)
+    rc = b.system.force("NonbondedForce").cutoff
+    edge = len(tree.query_pairs(rc + 2e-6)) - len(tree.query_pairs(rc - 2e-6))
+    print("worst displacement", stale["worst_displacement"], "stale census", pairs_stale, "fresh", pairs_fresh,
+          "oracle", d.pairs_geometric, "edge pairs", edge)
+    assert abs(pairs_stale - d.pairs_geometric) <= edge
+    assert abs(pairs_stale - pairs_fresh) <= edge
+    e.destroy()
+    fresh.destroy()
+
+
+def test_constraints_that_cannot_converge_raise_and_leave_the_step(prepared):
+    """A tolerance below f64 resolution leaves every SHAKE/RATTLE cluster
+    unconverged after its passes: md_run reports ConstraintsDiverged and the
+    step count stays where it was.  New positions clear the condition."""
+    b, pos, vel, step = prepared
+    e = _engine(b, pos, vel, step=step, constraintTolerance=1e-18)
+    with pytest.raises(native.ConstraintsDiverged) as err:
+        e.run(3)
+    print(err.value)
+    assert e.steps == step
+    e.destroy()
+    ok = _engine(b, pos, vel, step=step)
+    ok.run(3)
+    assert ok.steps == step + 3
+    ok.destroy()
+
+
+def test_list_overflow_during_a_run_stops_it_within_a_stride():
+    """A box compressed to about six times the starting density overflows
+    the cluster-pair list at the run's first rebuild: md_run raises
+    ListOverflow after the first stride, not after all its steps, and the
+    step count stays where it was."""
+    b = synthetic.config(1)
+    e = HipEngine(b.system, b.state, RunOptions(seed=3))
+    scale = 0.55
+    e.setBox(np.asarray(b.system.boxVectors) * scale)
+    e.setPositions(b.state.positions * scale)
+    with pytest.raises(native.ListOverflow) as err:
+        e.run(500)
+    print(err.value)
+    import re
+    at = int(re.search(r"at or before step (\d+)", str(err.value)).group(1))
+    assert at < 50, at
+    assert e.steps == 0
     e.destroy()
