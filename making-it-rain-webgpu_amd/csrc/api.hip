@@ -347,13 +347,10 @@ static void create(md_system *s, const md_system_desc *d, const md_run_options *
   if (s->opt.max_batch < 0 || s->opt.max_batch > 100000)
     throw md_error(MD_ERR_INVALID_ARGUMENT, "max_batch must be between 0 and 100000");
   if (s->opt.constraint_tolerance <= 0) s->opt.constraint_tolerance = 1e-6;
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
-    throw md_error(MD_ERR_NO_DEVICE, "no HIP device is visible; the engine has no CPU fallback");
-  if (o->device < 0 || o->device >= ndev) throw md_error(MD_ERR_INVALID_ARGUMENT, "no such HIP device");
-  s->device = o->device;
-  MD_HIP(hipSetDevice(s->device));
-  MD_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  // Everything up to the device state below is host-side validation and
+  // packing, so a System the engine refuses (ClusterTooLarge,
+  // CutoffTooLargeForBox, NotSmooth, bad indices) is refused the same way
+  // with or without a GPU.
   s->n = n;
   s->np = (n + 63) / 64 * 64;
   s->masses.assign(d->masses, d->masses + n);
@@ -440,6 +437,12 @@ static void create(md_system *s, const md_system_desc *d, const md_run_options *
         g = smooth_size(std::max(6, m));
       }
       if (g < MD_PME_ORDER) throw md_error(MD_ERR_INVALID_ARGUMENT, "PME mesh smaller than the spline order");
+      int r = g;
+      for (int f : {2, 3, 5, 7, 11, 13})
+        while (r % f == 0) r /= f;
+      if (r != 1)
+        throw md_error(MD_ERR_NOT_SMOOTH, "PME mesh size " + std::to_string(g) + " has a prime factor above 13 (" +
+                                              std::to_string(r) + " is left); mesh sizes must be 13-smooth");
       s->grid[a] = g;
     }
   }
@@ -518,6 +521,13 @@ static void create(md_system *s, const md_system_desc *d, const md_run_options *
   s->max_cluster_atoms = R.max_atoms;
 
   // device state
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    throw md_error(MD_ERR_NO_DEVICE, "no HIP device is visible; the engine has no CPU fallback");
+  if (o->device < 0 || o->device >= ndev) throw md_error(MD_ERR_INVALID_ARGUMENT, "no such HIP device");
+  s->device = o->device;
+  MD_HIP(hipSetDevice(s->device));
+  MD_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   int np = s->np;
   s->pos = md_alloc<int64_t>(s, 3 * (size_t)np);
   s->vel = md_alloc<float>(s, 3 * (size_t)np);

@@ -95,15 +95,59 @@ struct md_item {
   int64_t x[MAXA][3];   // positions at the start of the step
   float v[MAXA][3];
   int64_t tr[MAXA][3];  // trial
-  int64_t un[MAXA][3];  // unconstrained trial
+  int64_t dc[MAXA][3];  // what the constraints moved the trial by (trial - unconstrained trial)
 };
+
+// Most constraints a cluster of MAXA atoms can hold: every pair, at most
+// MD_MAX_CLUSTER_CONSTRAINTS.
+template <int MAXA>
+struct md_max_constraints {
+  static constexpr int value = MAXA * (MAXA - 1) / 2 < MD_MAX_CLUSTER_CONSTRAINTS ? MAXA * (MAXA - 1) / 2
+                                                                                  : MD_MAX_CLUSTER_CONSTRAINTS;
+};
+
+// Private arrays are indexed by compile-time constants only: every loop over
+// a cluster's atoms or constraints is unrolled to its bound and guarded, and
+// a constraint's runtime atom index selects its atom through a chain of
+// selects.  A runtime index into a private array would put the array in
+// scratch memory.
+__device__ __forceinline__ uint64_t md_bits(int64_t v) { return (uint64_t)v; }
+__device__ __forceinline__ uint64_t md_bits(double v) { return (uint64_t)__double_as_longlong(v); }
+__device__ __forceinline__ uint32_t md_bits(float v) { return __float_as_uint(v); }
+__device__ __forceinline__ void md_unbits(uint64_t b, int64_t &v) { v = (int64_t)b; }
+__device__ __forceinline__ void md_unbits(uint64_t b, double &v) { v = __longlong_as_double((long long)b); }
+__device__ __forceinline__ void md_unbits(uint32_t b, float &v) { v = __uint_as_float(b); }
+
+// The element `i` of a register array as an OR of masked bit patterns: a
+// chain of selects would be folded back into an indexed load.
+template <int N, typename T>
+__device__ __forceinline__ T md_pick(const T (&a)[N][3], int i, int d) {
+  decltype(md_bits(a[0][d])) r = 0;
+#pragma unroll
+  for (int k = 0; k < N; ++k) r |= md_bits(a[k][d]) & (0 - (decltype(r))(i == k));
+  T v;
+  md_unbits(r, v);
+  return v;
+}
+template <int N, typename T>
+__device__ __forceinline__ T md_pick1(const T (&a)[N], int i) {
+  decltype(md_bits(a[0])) r = 0;
+#pragma unroll
+  for (int k = 0; k < N; ++k) r |= md_bits(a[k]) & (0 - (decltype(r))(i == k));
+  T v;
+  md_unbits(r, v);
+  return v;
+}
 
 // kick, and for LFMiddle nothing else before RATTLE
 template <int MAXA>
-__device__ void md_load_kick(const md_int_args &A, md_item<MAXA> &it, const float *vcom) {
-  for (int a = 0; a < it.n; ++a) {
+__device__ __forceinline__ void md_load_kick(const md_int_args &A, md_item<MAXA> &it, const float *vcom) {
+#pragma unroll
+  for (int a = 0; a < MAXA; ++a) {
+    if (a >= it.n) break;
     int i = it.atom[a];
     it.im[a] = A.inv_mass[i];
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
       it.x[a][d] = A.pos[d * A.np + i];
       float v = A.vel[d * A.np + i];
@@ -120,25 +164,33 @@ __device__ void md_load_kick(const md_int_args &A, md_item<MAXA> &it, const floa
 
 // half drift, thermostat, half drift (LFMiddle) or a whole drift (Verlet)
 template <int MAXA>
-__device__ void md_drift(const md_int_args &A, md_item<MAXA> &it, uint32_t step) {
-  for (int a = 0; a < it.n; ++a) {
+__device__ __forceinline__ void md_drift(const md_int_args &A, md_item<MAXA> &it, uint32_t step) {
+#pragma unroll
+  for (int a = 0; a < MAXA; ++a) {
+    if (a >= it.n) break;
     int i = it.atom[a];
     if (it.im[a] == 0.0f) {
-      for (int d = 0; d < 3; ++d) it.tr[a][d] = it.un[a][d] = it.x[a][d];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        it.tr[a][d] = it.x[a][d];
+        it.dc[a][d] = 0;
+      }
       continue;
     }
     if (A.integrator == 1) {
+#pragma unroll
       for (int d = 0; d < 3; ++d) {
         float left = 0.0f;
         it.tr[a][d] = md_displace(it.x[a][d], it.v[a][d] * A.dt + A.resid[d * A.np + i], left);
         A.resid[d * A.np + i] = left;
-        it.un[a][d] = it.tr[a][d];
+        it.dc[a][d] = 0;
       }
       continue;
     }
     float dev[3] = {0.0f, 0.0f, 0.0f};
     if (A.noise_scale != 0.0f) md_thermostat_normals(A.seed, step, (uint32_t)i, dev);
     float noise = A.noise_scale * sqrtf(A.kt * it.im[a]);
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
       float left = A.resid[d * A.np + i];
       int64_t moved = md_displace(it.x[a][d], it.v[a][d] * A.half_dt + left, left);
@@ -147,27 +199,33 @@ __device__ void md_drift(const md_int_args &A, md_item<MAXA> &it, uint32_t step)
       moved = md_displace(moved, th * A.half_dt + left, left);
       A.resid[d * A.np + i] = left;
       it.tr[a][d] = moved;
-      it.un[a][d] = moved;
+      it.dc[a][d] = 0;
     }
   }
 }
 
 // velocity from the constraint correction, commit, momentum, displacement
 template <int MAXA>
-__device__ void md_finish(const md_int_args &A, md_item<MAXA> &it, double *mom, float &worst, uint32_t &failed) {
-  for (int a = 0; a < it.n; ++a) {
+__device__ __forceinline__ void md_finish(const md_int_args &A, md_item<MAXA> &it, double *mom, float &worst, uint32_t &failed) {
+#pragma unroll
+  for (int a = 0; a < MAXA; ++a) {
+    if (a >= it.n) break;
     int i = it.atom[a];
     if (it.im[a] != 0.0f) {
+#pragma unroll
       for (int d = 0; d < 3; ++d) {
-        if (A.integrator == 1)
-          it.v[a][d] = A.inv_dt * md_from_fixed(it.tr[a][d] - it.x[a][d]);
-        else
-          it.v[a][d] = it.v[a][d] + A.inv_dt * md_from_fixed(it.tr[a][d] - it.un[a][d]);
+        // Verlet: the whole displacement over dt; LFMiddle: the constraint
+        // correction over dt added (values selected, not arrays)
+        const bool verlet = A.integrator == 1;
+        const int64_t moved = verlet ? it.tr[a][d] - it.x[a][d] : it.dc[a][d];
+        const float base = verlet ? 0.0f : it.v[a][d];
+        it.v[a][d] = base + A.inv_dt * md_from_fixed(moved);
         A.pos[d * A.np + i] = it.tr[a][d];
       }
     }
     float m = A.mass[i];
     float d2 = 0.0f;
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
       A.vel[d * A.np + i] = it.v[a][d];
       if (!isfinite(it.v[a][d])) failed |= 4u;
@@ -187,29 +245,40 @@ __device__ __forceinline__ double md_fx(int64_t v) { return (double)v * 2.328306
 // bond vectors, divided by the reference squared length
 // (false: not converged within MD_MAX_CONSTRAINT_ITERATIONS passes)
 template <int MAXA>
-__device__ bool md_rattle(const md_int_args &A, md_item<MAXA> &it, int nc, const int4 *cons) {
-  double rv[MD_MAX_CLUSTER_CONSTRAINTS][3], l2[MD_MAX_CLUSTER_CONSTRAINTS], mu[MD_MAX_CLUSTER_CONSTRAINTS];
+__device__ __forceinline__ bool md_rattle(const md_int_args &A, md_item<MAXA> &it, int nc, const int4 *cons) {
+  constexpr int MAXC = md_max_constraints<MAXA>::value;
+  double rv[MAXC][3], l2[MAXC], mu[MAXC];
   double v[MAXA][3];
-  for (int a = 0; a < it.n; ++a)
+#pragma unroll
+  for (int a = 0; a < MAXA; ++a)
+#pragma unroll
     for (int d = 0; d < 3; ++d) v[a][d] = it.v[a][d];
-  for (int c = 0; c < nc; ++c) {
+#pragma unroll
+  for (int c = 0; c < MAXC; ++c) {
+    l2[c] = 0.0;
+    mu[c] = 0.0;
+    if (c >= nc) continue;
     int li = cons[c].x, lj = cons[c].y;
-    l2[c] = 0;
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
-      rv[c][d] = md_fx(it.x[li][d] - it.x[lj][d]);
+      rv[c][d] = md_fx(md_pick(it.x, li, d) - md_pick(it.x, lj, d));
       l2[c] += rv[c][d] * rv[c][d];
     }
-    double sum = (double)it.im[li] + (double)it.im[lj];
+    double sum = (double)md_pick1(it.im, li) + (double)md_pick1(it.im, lj);
     mu[c] = sum == 0.0 ? 0.0 : 0.5 / sum;
   }
-  double delta[MD_MAX_CLUSTER_CONSTRAINTS];
+  double delta[MAXC];
   bool converged = false;
   for (int iter = 0; iter < MD_MAX_CONSTRAINT_ITERATIONS; ++iter) {
     int conv = 0;
-    for (int c = 0; c < nc; ++c) {
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c) {
+      delta[c] = 0.0;
+      if (c >= nc) continue;
       int li = cons[c].x, lj = cons[c].y;
       double proj = 0;
-      for (int d = 0; d < 3; ++d) proj += (v[li][d] - v[lj][d]) * rv[c][d];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) proj += (md_pick(v, li, d) - md_pick(v, lj, d)) * rv[c][d];
       delta[c] = l2[c] == 0.0 ? 0.0 : -2.0 * mu[c] * proj / l2[c];
       conv += fabs(delta[c]) <= A.tol;
     }
@@ -217,16 +286,24 @@ __device__ bool md_rattle(const md_int_args &A, md_item<MAXA> &it, int nc, const
       converged = true;
       break;
     }
-    for (int c = 0; c < nc; ++c) {
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c) {
+      if (c >= nc) continue;
       int li = cons[c].x, lj = cons[c].y;
+#pragma unroll
       for (int d = 0; d < 3; ++d) {
         double dr = rv[c][d] * delta[c];
-        v[li][d] += dr * it.im[li];
-        v[lj][d] -= dr * it.im[lj];
+#pragma unroll
+        for (int a = 0; a < MAXA; ++a) {
+          if (a == li) v[a][d] += dr * it.im[a];
+          if (a == lj) v[a][d] -= dr * it.im[a];
+        }
       }
     }
   }
-  for (int a = 0; a < it.n; ++a)
+#pragma unroll
+  for (int a = 0; a < MAXA; ++a)
+#pragma unroll
     for (int d = 0; d < 3; ++d) it.v[a][d] = (float)v[a][d];
   return converged || nc == 0;
 }
@@ -236,78 +313,109 @@ __device__ bool md_rattle(const md_int_args &A, md_item<MAXA> &it, int nc, const
 // *passes receives the update passes applied, or -1 when the cluster did not
 // converge within MD_MAX_CONSTRAINT_ITERATIONS.
 template <int MAXA>
-__device__ void md_shake(const md_int_args &A, md_item<MAXA> &it, int nc, const int4 *cons,
-                         const float *lam0 = nullptr, double *lam_total = nullptr, int *passes = nullptr) {
-  double rv[MD_MAX_CLUSTER_CONSTRAINTS][3], mu[MD_MAX_CLUSTER_CONSTRAINTS], t2[MD_MAX_CLUSTER_CONSTRAINTS];
+__device__ __forceinline__ void md_shake(const md_int_args &A, md_item<MAXA> &it, int nc, const int4 *cons,
+                                         bool warm = false, const float *lam0 = nullptr,
+                                         double *lam_total = nullptr, int *passes = nullptr) {
+  constexpr int MAXC = md_max_constraints<MAXA>::value;
+  double rv[MAXC][3], mu[MAXC], t2[MAXC];
   double t[MAXA][3], moved[MAXA][3];
-  for (int a = 0; a < it.n; ++a)
+#pragma unroll
+  for (int a = 0; a < MAXA; ++a)
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
       t[a][d] = md_fx(it.tr[a][d] - it.x[0][d]);
       moved[a][d] = 0.0;
     }
-  for (int c = 0; c < nc; ++c) {
+#pragma unroll
+  for (int c = 0; c < MAXC; ++c) {
+    mu[c] = 0.0;
+    t2[c] = 0.0;
+    if (c >= nc) continue;
     int li = cons[c].x, lj = cons[c].y;
-    for (int d = 0; d < 3; ++d) rv[c][d] = md_fx(it.x[li][d] - it.x[lj][d]);
-    double sum = (double)it.im[li] + (double)it.im[lj];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) rv[c][d] = md_fx(md_pick(it.x, li, d) - md_pick(it.x, lj, d));
+    double sum = (double)md_pick1(it.im, li) + (double)md_pick1(it.im, lj);
     mu[c] = sum == 0.0 ? 0.0 : 0.5 / sum;
     double dist = __hiloint2double(cons[c].w, cons[c].z);
     t2[c] = dist * dist;
   }
   double lower = 1.0 - 2.0 * A.tol + A.tol * A.tol, upper = 1.0 + 2.0 * A.tol + A.tol * A.tol;
-  double delta[MD_MAX_CLUSTER_CONSTRAINTS];
+  double delta[MAXC];
   int applied = 0;
-  for (int c = 0; c < nc; ++c) delta[c] = 0.0;
-  if (lam0) {
+#pragma unroll
+  for (int c = 0; c < MAXC; ++c) delta[c] = 0.0;
+  if (warm) {
     // The predicted multipliers (in units of each constraint's reduced-mass
     // factor mu), applied as one update; the passes below correct whatever
     // the prediction missed.  A prediction that leaves the constraints
     // further from their lengths than the unconstrained trial positions is
     // dropped, so a poor network never costs SHAKE more than a cold start.
-    auto worst = [&](const double (*p)[3]) {
+    auto worst = [&](const double (&p)[MAXA][3]) {
       double w = 0.0;
-      for (int c = 0; c < nc; ++c) {
+#pragma unroll
+      for (int c = 0; c < MAXC; ++c) {
+        if (c >= nc) continue;
         int li = cons[c].x, lj = cons[c].y;
-        double x = p[li][0] - p[lj][0], y = p[li][1] - p[lj][1], z = p[li][2] - p[lj][2];
+        double x = md_pick(p, li, 0) - md_pick(p, lj, 0), y = md_pick(p, li, 1) - md_pick(p, lj, 1),
+               z = md_pick(p, li, 2) - md_pick(p, lj, 2);
         w = fmax(w, fabs((x * x + y * y + z * z) / t2[c] - 1.0));
       }
       return w;
     };
     double tw[MAXA][3];
-    for (int a = 0; a < it.n; ++a)
+#pragma unroll
+    for (int a = 0; a < MAXA; ++a)
+#pragma unroll
       for (int d = 0; d < 3; ++d) tw[a][d] = t[a][d];
     bool finite = true;
-    for (int c = 0; c < nc && c < MD_WS_MAXC; ++c) {
+#pragma unroll
+    for (int c = 0; c < MAXC && c < MD_WS_MAXC; ++c) {
+      if (c >= nc) continue;
       delta[c] = mu[c] * (double)lam0[c];
       finite &= isfinite(delta[c]);
     }
     if (finite) {
-      for (int c = 0; c < nc; ++c) {
+#pragma unroll
+      for (int c = 0; c < MAXC; ++c) {
+        if (c >= nc) continue;
         int li = cons[c].x, lj = cons[c].y;
+#pragma unroll
         for (int d = 0; d < 3; ++d) {
           double dr = rv[c][d] * delta[c];
-          tw[li][d] += dr * it.im[li];
-          tw[lj][d] -= dr * it.im[lj];
+#pragma unroll
+          for (int a = 0; a < MAXA; ++a) {
+            if (a == li) tw[a][d] += dr * it.im[a];
+            if (a == lj) tw[a][d] -= dr * it.im[a];
+          }
         }
       }
     }
     if (finite && worst(tw) < worst(t)) {
-      for (int a = 0; a < it.n; ++a)
+#pragma unroll
+      for (int a = 0; a < MAXA; ++a)
+#pragma unroll
         for (int d = 0; d < 3; ++d) {
           moved[a][d] = tw[a][d] - t[a][d];
           t[a][d] = tw[a][d];
         }
     } else {
-      for (int c = 0; c < nc; ++c) delta[c] = 0.0;
+#pragma unroll
+      for (int c = 0; c < MAXC; ++c) delta[c] = 0.0;
     }
   }
-  double total[MD_MAX_CLUSTER_CONSTRAINTS];
-  for (int c = 0; c < nc; ++c) total[c] = delta[c];
+  double total[MAXC];
+#pragma unroll
+  for (int c = 0; c < MAXC; ++c) total[c] = delta[c];
   bool converged = nc == 0;
   for (int iter = 0; iter < MD_MAX_CONSTRAINT_ITERATIONS; ++iter) {
     int conv = 0;
-    for (int c = 0; c < nc; ++c) {
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c) {
+      delta[c] = 0.0;
+      if (c >= nc) continue;
       int li = cons[c].x, lj = cons[c].y;
-      double x = t[li][0] - t[lj][0], y = t[li][1] - t[lj][1], z = t[li][2] - t[lj][2];
+      double x = md_pick(t, li, 0) - md_pick(t, lj, 0), y = md_pick(t, li, 1) - md_pick(t, lj, 1),
+             z = md_pick(t, li, 2) - md_pick(t, lj, 2);
       double l2 = x * x + y * y + z * z;
       double proj = x * rv[c][0] + y * rv[c][1] + z * rv[c][2];
       delta[c] = proj == 0.0 ? 0.0 : mu[c] * (t2[c] - l2) / proj;
@@ -318,23 +426,41 @@ __device__ void md_shake(const md_int_args &A, md_item<MAXA> &it, int nc, const 
       break;
     }
     ++applied;
-    for (int c = 0; c < nc; ++c) {
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c) {
+      if (c >= nc) continue;
       int li = cons[c].x, lj = cons[c].y;
       total[c] += delta[c];
+#pragma unroll
       for (int d = 0; d < 3; ++d) {
         double dr = rv[c][d] * delta[c];
-        t[li][d] += dr * it.im[li];
-        t[lj][d] -= dr * it.im[lj];
-        moved[li][d] += dr * it.im[li];
-        moved[lj][d] -= dr * it.im[lj];
+#pragma unroll
+        for (int a = 0; a < MAXA; ++a) {
+          if (a == li) {
+            t[a][d] += dr * it.im[a];
+            moved[a][d] += dr * it.im[a];
+          }
+          if (a == lj) {
+            t[a][d] -= dr * it.im[a];
+            moved[a][d] -= dr * it.im[a];
+          }
+        }
       }
     }
   }
-  for (int a = 0; a < it.n; ++a)
-    for (int d = 0; d < 3; ++d) it.tr[a][d] += (int64_t)rint(moved[a][d] * 4294967296.0);
+#pragma unroll
+  for (int a = 0; a < MAXA; ++a)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int64_t by = (int64_t)rint(moved[a][d] * 4294967296.0);
+      it.tr[a][d] += by;
+      it.dc[a][d] += by;
+    }
   if (passes) *passes = converged ? applied : -1;
-  if (lam_total)
-    for (int c = 0; c < nc && c < MD_WS_MAXC; ++c) lam_total[c] = mu[c] == 0.0 ? 0.0 : total[c] / mu[c];
+  if (lam_total) {
+#pragma unroll
+    for (int c = 0; c < MAXC && c < MD_WS_MAXC; ++c) lam_total[c] = mu[c] == 0.0 ? 0.0 : total[c] / mu[c];
+  }
 }
 
 // The network's inputs for one cluster of up to MD_WS_MAXC constraints, from
@@ -346,37 +472,46 @@ __device__ void md_shake(const md_int_args &A, md_item<MAXA> &it, int nc, const 
 // sums) times the cosine between their reference vectors.  All invariant
 // under rotation and translation.
 template <int MAXA>
-__device__ void md_ws_features(const md_item<MAXA> &it, int nc, const int4 *cons, float *f) {
+__device__ __forceinline__ void md_ws_features(const md_item<MAXA> &it, int nc, const int4 *cons, float *f) {
+#pragma unroll
   for (int k = 0; k < MD_WS_F; ++k) f[k] = 0.0f;
   double rv[MD_WS_MAXC][3], d2[MD_WS_MAXC], isum[MD_WS_MAXC];
   int n = min(nc, MD_WS_MAXC);
-  for (int c = 0; c < n; ++c) {
+#pragma unroll
+  for (int c = 0; c < MD_WS_MAXC; ++c) {
+    d2[c] = isum[c] = 0.0;
+    rv[c][0] = rv[c][1] = rv[c][2] = 0.0;
+    if (c >= n) continue;
     int li = cons[c].x, lj = cons[c].y;
     double rp[3], l2 = 0.0, pr = 0.0;
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
-      rv[c][d] = md_fx(it.x[li][d] - it.x[lj][d]);
-      rp[d] = md_fx(it.tr[li][d] - it.tr[lj][d]);
+      rv[c][d] = md_fx(md_pick(it.x, li, d) - md_pick(it.x, lj, d));
+      rp[d] = md_fx(md_pick(it.tr, li, d) - md_pick(it.tr, lj, d));
       l2 += rp[d] * rp[d];
       pr += rp[d] * rv[c][d];
     }
     double dist = __hiloint2double(cons[c].w, cons[c].z);
     d2[c] = dist * dist;
-    isum[c] = (double)it.im[li] + (double)it.im[lj];
+    const double imi = md_pick1(it.im, li), imj = md_pick1(it.im, lj);
+    isum[c] = imi + imj;
     f[4 * c + 0] = (float)(MD_WS_SCALE * (d2[c] - l2) / d2[c]);
     f[4 * c + 1] = (float)(MD_WS_SCALE * (pr / d2[c] - 1.0));
-    f[4 * c + 2] = isum[c] > 0.0 ? (float)(it.im[li] / isum[c]) : 0.0f;
+    f[4 * c + 2] = isum[c] > 0.0 ? (float)(imi / isum[c]) : 0.0f;
     f[4 * c + 3] = 1.0f;
   }
   int pair = 0;
+#pragma unroll
   for (int c = 0; c < MD_WS_MAXC; ++c)
+#pragma unroll
     for (int e = c + 1; e < MD_WS_MAXC; ++e, ++pair) {
       if (e >= n) continue;
       int ci = cons[c].x, cj = cons[c].y, ei = cons[e].x, ej = cons[e].y;
       double shared = 0.0;
-      if (ci == ei) shared += it.im[ci];
-      if (cj == ej) shared += it.im[cj];
-      if (ci == ej) shared -= it.im[ci];
-      if (cj == ei) shared -= it.im[cj];
+      if (ci == ei) shared += md_pick1(it.im, ci);
+      if (cj == ej) shared += md_pick1(it.im, cj);
+      if (ci == ej) shared -= md_pick1(it.im, ci);
+      if (cj == ei) shared -= md_pick1(it.im, cj);
       if (shared == 0.0 || isum[c] == 0.0 || isum[e] == 0.0) continue;
       double dot = rv[c][0] * rv[e][0] + rv[c][1] * rv[e][1] + rv[c][2] * rv[e][2];
       f[16 + pair] = (float)(shared / sqrt(isum[c] * isum[e]) * dot / sqrt(d2[c] * d2[e]));
@@ -385,7 +520,7 @@ __device__ void md_ws_features(const md_item<MAXA> &it, int nc, const int4 *cons
 
 // SETTLE positions (src/reference/settle.ts:31-196): rigid rotation about the
 // centre of mass in the frame of the reference triangle
-__device__ void md_settle_positions(md_item<3> &it, double d1, double d2, const float *mass3) {
+__device__ __forceinline__ void md_settle_positions(md_item<3> &it, double d1, double d2, const float *mass3) {
   double m0 = mass3[0], m1 = mass3[1], m2 = mass3[2];
   double xp0[3], xp1[3], xp2[3], b0[3], c0[3];
   for (int k = 0; k < 3; ++k) {
@@ -450,14 +585,20 @@ __device__ void md_settle_positions(md_item<3> &it, double d1, double d2, const 
     double a3 = ex[k] * a3d[0] + ey[k] * a3d[1] + ez[k] * a3d[2];
     double b3 = ex[k] * b3d[0] + ey[k] * b3d[1] + ez[k] * b3d[2];
     double c3 = ex[k] * c3d[0] + ey[k] * c3d[1] + ez[k] * c3d[2];
-    it.tr[0][k] = it.x[0][k] + (int64_t)rint((com[k] + a3) * 4294967296.0);
-    it.tr[1][k] = it.x[1][k] + (int64_t)rint((com[k] + b3 - b0[k]) * 4294967296.0);
-    it.tr[2][k] = it.x[2][k] + (int64_t)rint((com[k] + c3 - c0[k]) * 4294967296.0);
+    const int64_t t0 = it.x[0][k] + (int64_t)rint((com[k] + a3) * 4294967296.0);
+    const int64_t t1 = it.x[1][k] + (int64_t)rint((com[k] + b3 - b0[k]) * 4294967296.0);
+    const int64_t t2 = it.x[2][k] + (int64_t)rint((com[k] + c3 - c0[k]) * 4294967296.0);
+    it.dc[0][k] += t0 - it.tr[0][k];
+    it.dc[1][k] += t1 - it.tr[1][k];
+    it.dc[2][k] += t2 - it.tr[2][k];
+    it.tr[0][k] = t0;
+    it.tr[1][k] = t1;
+    it.tr[2][k] = t2;
   }
 }
 
 // SETTLE velocities (src/reference/settle.ts:198-284): exact 3x3 impulse solve
-__device__ void md_settle_velocities(md_item<3> &it, const float *mass3) {
+__device__ __forceinline__ void md_settle_velocities(md_item<3> &it, const float *mass3) {
   double mA = mass3[0], mB = mass3[1], mC = mass3[2];
   double iA = mA == 0 ? 0 : 1 / mA, iB = mB == 0 ? 0 : 1 / mB, iC = mC == 0 ? 0 : 1 / mC;
   double e[3][3];
@@ -504,12 +645,14 @@ __device__ __forceinline__ void md_com_velocity(const md_int_args &A, uint32_t s
   vcom[0] = vcom[1] = vcom[2] = 0.0f;
   if (A.cmm_frequency > 0 && step % (uint32_t)A.cmm_frequency == 0) {
     const int64_t *p = A.ctl->momentum[step & 1];
+#pragma unroll
     for (int d = 0; d < 3; ++d) vcom[d] = (float)(md_fixed_to_double_dev(p[d]) * (double)A.inv_total_mass);
   }
 }
 
 __device__ __forceinline__ void md_reduce_out(const md_int_args &A, uint32_t step, const double *mom, float worst) {
   // momentum for the next step's COM removal, and the displacement monitor
+#pragma unroll
   for (int d = 0; d < 3; ++d) {
     int64_t f = md_wave_sum_i64((int64_t)rint(mom[d] * 4294967296.0));
     if ((threadIdx.x & 63) == 0) md_atomic_add_i64(&A.ctl->momentum[(step + 1) & 1][d], f);
@@ -548,7 +691,8 @@ __global__ void __launch_bounds__(256) k_integrate(const md_int_args *__restrict
   it.n = 0;
   int nc = 0;
   uint32_t failed = 0;  // control-block divergence bits this thread raises
-  int4 cons[KIND == 2 ? MD_MAX_CLUSTER_CONSTRAINTS : 1];
+  constexpr int MAXC = KIND == 2 ? md_max_constraints<MAXA>::value : 1;
+  int4 cons[MAXC];
   if (live) {
     if (KIND == 0) {
       it.n = 1;
@@ -561,14 +705,17 @@ __global__ void __launch_bounds__(256) k_integrate(const md_int_args *__restrict
       it.atom[2] = w.z;
     } else {
       it.n = A.ccl_natoms[t];
+#pragma unroll
       for (int a = 0; a < MAXA; ++a) it.atom[a] = a < it.n ? A.ccl_atoms[t * MD_MAX_CLUSTER_ATOMS + a] : 0;
       nc = A.ccl_ncons[t];
-      for (int c = 0; c < (KIND == 2 ? MD_MAX_CLUSTER_CONSTRAINTS : 1); ++c)
+#pragma unroll
+      for (int c = 0; c < MAXC; ++c)
         cons[c] = c < nc ? A.ccl_cons[t * MD_MAX_CLUSTER_CONSTRAINTS + c] : make_int4(0, 0, 0, 0);
     }
     md_load_kick(A, it, vcom);
     if constexpr (KIND == 1) {
       float m3[3];
+#pragma unroll
       for (int a = 0; a < 3; ++a) m3[a] = A.mass[it.atom[a]];
       if (A.integrator == 0) md_settle_velocities(it, m3);
       md_drift(A, it, step);
@@ -606,7 +753,7 @@ __global__ void __launch_bounds__(256) k_integrate(const md_int_args *__restrict
       double total[MD_WS_MAXC];
       float f[MD_WS_F];
       if (MODE == 1 && covered) md_ws_features(it, nc, cons, f);
-      md_shake(A, it, nc, cons, warm ? lam0 : nullptr, MODE == 1 ? total : nullptr, &passes);
+      md_shake(A, it, nc, cons, warm, lam0, MODE == 1 ? total : nullptr, &passes);
       if (passes < 0) {
         failed |= 1u;
         passes = MD_MAX_CONSTRAINT_ITERATIONS;
@@ -648,7 +795,8 @@ __global__ void __launch_bounds__(256) k_constrain(const md_int_args *__restrict
   if (t >= count) return;
   md_item<MAXA> it;
   int nc = 0;
-  int4 cons[KIND == 2 ? MD_MAX_CLUSTER_CONSTRAINTS : 1];
+  constexpr int MAXC = KIND == 2 ? md_max_constraints<MAXA>::value : 1;
+  int4 cons[MAXC];
   if (KIND == 1) {
     int4 w = A.water_atoms[t];
     it.n = 3;
@@ -657,14 +805,25 @@ __global__ void __launch_bounds__(256) k_constrain(const md_int_args *__restrict
     it.atom[2] = w.z;
   } else {
     it.n = A.ccl_natoms[t];
+#pragma unroll
     for (int a = 0; a < MAXA; ++a) it.atom[a] = a < it.n ? A.ccl_atoms[t * MD_MAX_CLUSTER_ATOMS + a] : 0;
     nc = A.ccl_ncons[t];
-    for (int c = 0; c < (KIND == 2 ? MD_MAX_CLUSTER_CONSTRAINTS : 1); ++c)
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c)
       cons[c] = c < nc ? A.ccl_cons[t * MD_MAX_CLUSTER_CONSTRAINTS + c] : make_int4(0, 0, 0, 0);
   }
-  for (int a = 0; a < it.n; ++a) {
+#pragma unroll
+  for (int a = 0; a < MAXA; ++a) {
+    it.im[a] = 0.0f;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      it.x[a][d] = it.tr[a][d] = it.dc[a][d] = 0;
+      it.v[a][d] = 0.0f;
+    }
+    if (a >= it.n) continue;
     int i = it.atom[a];
     it.im[a] = A.inv_mass[i];
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
       it.x[a][d] = it.tr[a][d] = A.pos[d * A.np + i];
       it.v[a][d] = A.vel[d * A.np + i];
@@ -672,6 +831,7 @@ __global__ void __launch_bounds__(256) k_constrain(const md_int_args *__restrict
   }
   if constexpr (KIND == 1) {
     float m3[3];
+#pragma unroll
     for (int a = 0; a < 3; ++a) m3[a] = A.mass[it.atom[a]];
     if (WHAT == 0) {
       float2 dd = A.water_dist[t];
@@ -685,8 +845,11 @@ __global__ void __launch_bounds__(256) k_constrain(const md_int_args *__restrict
     else
       md_rattle(A, it, nc, cons);
   }
-  for (int a = 0; a < it.n; ++a) {
+#pragma unroll
+  for (int a = 0; a < MAXA; ++a) {
+    if (a >= it.n) break;
     int i = it.atom[a];
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
       if (WHAT == 0)
         A.pos[d * A.np + i] = it.tr[a][d];

@@ -130,8 +130,12 @@ struct md_pair_args {
   int64_t *census;  // in-cutoff pairs (md_control)
   int64_t *blocks;  // 8x8 cluster-pair blocks evaluated (md_control)
   int nslots, nclusters;
-  float cutoff2, alpha, beta, exp_scale;
-  float prune2;  // cutoff^2 (1 + 1e-5): a block whose boxes are this far apart has no pair within the cutoff
+  // in the pair kernel's frame units (see md_pair): cutoff^2 in 2^-52 nm^2;
+  // alpha/2 and 2 alpha / sqrt(pi) times 2^-26; -alpha^2 / ln 2 times 2^-52
+  float cutoff2, half_alpha, beta, exp_scale;
+  // cutoff^2 (1 + 1e-5) in 2^-52 nm^2: a block whose boxes are this far apart
+  // has no pair within the cutoff
+  float prune2;
 };
 
 // Everything the nonbonded kernels of one replica read: its table entry.
@@ -171,6 +175,11 @@ __device__ __forceinline__ const md_nb_args &md_nb_entry(const md_nb_args *T, co
 #define MD_FRAME_SHIFT 6  // Q32.32 -> 2^-26 nm
 #define MD_FRAME_UNIT 1.4901161193847656e-08f   // 2^-26 nm
 #define MD_FRAME_UNIT2 2.220446049250313e-16f   // 2^-52 nm^2
+// the pair kernel's per-atom scalings (see md_pair): charge (with sqrt K)
+// 2^26, sigma/2 2^26, 2 sqrt(eps) 2^13
+#define MD_PAIR_Q_SCALE 67108864.0f
+#define MD_PAIR_S_SCALE 67108864.0f
+#define MD_PAIR_E_SCALE 8192.0f
 
 __device__ __forceinline__ int md_frame_coord(int64_t v, bool &bad) {
   int64_t u = (v + (int64_t(1) << (MD_FRAME_SHIFT - 1))) >> MD_FRAME_SHIFT;
@@ -864,18 +873,22 @@ __global__ void __launch_bounds__(64 * MD_MASK_GROUP) k_nb_j_fill(const md_nb_ar
 // The pair kernel
 // ---------------------------------------------------------------------------
 
-// one pair from lane's i atom to the entry's j atom at squared distance r2
-// (nm^2); returns the F/r factor times the frame unit (so F_i = -fs * d with
-// d in frame units gives kJ/mol/nm) and accumulates the pair energy
+// One pair from lane's i atom to the entry's j atom, counted when `inc` (not
+// excluded, inside the cutoff).  Everything is in frame units: r2 in 2^-52
+// nm^2 (the integer difference squared), the charges carry 2^26, sigma/2
+// 2^26 and 2 sqrt(eps) 2^13, and the constants their powers of two, so the
+// result is the F/r factor per frame unit (F_i = -fs * d with d in frame
+// units gives kJ/mol/nm) and the energy carries 2^26.  Scaling by powers of
+// two is exact in f32, so this is the nm evaluation bit for bit, without
+// its two unit conversions per pair.
 template <bool ENERGY>
 __device__ __forceinline__ float md_pair(float r2, float qi, float si, float ei, float qj, float sj, float ej,
-                                         bool allowed, const md_pair_args &A, float &en, int &count) {
-  bool inc = allowed && r2 < A.cutoff2;
+                                         bool inc, const md_pair_args &A, float &en, int &count) {
   float rinv = __builtin_amdgcn_rsqf(r2);
   float rinv2 = rinv * rinv;
   float r = r2 * rinv;
   float ex = __builtin_amdgcn_exp2f(A.exp_scale * r2);  // exp(-alpha^2 r^2)
-  float t = __builtin_amdgcn_rcpf(fmaf(0.5f * A.alpha, r, 1.0f));
+  float t = __builtin_amdgcn_rcpf(fmaf(A.half_alpha, r, 1.0f));
   float erfc_v = ex * md_erfcx_poly(t);
   float qq = qi * qj;
   float fc = qq * fmaf(erfc_v, rinv, A.beta * ex) * rinv2;
@@ -885,7 +898,7 @@ __device__ __forceinline__ float md_pair(float r2, float qi, float si, float ei,
   float s6 = s2 * s2 * s2;
   float s12 = s6 * s6;
   float flj = eps * fmaf(12.0f, s12, -6.0f * s6) * rinv2;
-  float fs = inc ? (fc + flj) * MD_FRAME_UNIT : 0.0f;
+  float fs = inc ? fc + flj : 0.0f;
   if (ENERGY) {
     float e = fmaf(qq * erfc_v, rinv, eps * (s12 - s6));
     en += inc ? e : 0.0f;
@@ -894,11 +907,68 @@ __device__ __forceinline__ float md_pair(float r2, float qi, float si, float ei,
   return fs;
 }
 
-// gap between boxes [ilo, ihi] and [jlo, jhi] on one axis, in nm (frame
-// units; exact integer differences for boxes within 32 nm of each other)
+// Loads of wave-uniform addresses from list data no kernel of this launch
+// writes: through the constant address space they become scalar loads into
+// SGPRs (the list is built by earlier launches, so the scalar cache sees it).
+template <typename T>
+__device__ __forceinline__ T md_uniform(const T *p) {
+  return *(const __attribute__((address_space(4))) T *)p;
+}
+// an int4 of which only x, y, z are used, as two 64-bit scalar loads (there
+// is no three-dword scalar load, and a narrowed one would become a vector load)
+__device__ __forceinline__ int4 md_uniform_xyz(const int4 *p) {
+  const uint64_t a = md_uniform((const uint64_t *)p), b = md_uniform((const uint64_t *)p + 1);
+  return make_int4((int)(uint32_t)a, (int)(uint32_t)(a >> 32), (int)(uint32_t)b, 0);
+}
+
+// DPP lane moves (row = 16 lanes): quad_perm [1,0,3,2] and [2,3,0,1] give
+// lane l the value of l^1 and l^2; row_ror:n the value of lane (l+n) mod 16
+// of its row
+template <int CTRL>
+__device__ __forceinline__ float md_dpp(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, false));
+}
+#define MD_DPP_XOR1 0xB1
+#define MD_DPP_XOR2 0x4E
+#define MD_DPP_ROR4 0x124
+#define MD_DPP_ROR8 0x128
+#define MD_DPP_ROR12 0x12C
+
+// x(l) + x(l^16) and x(l) + x(l^32) on every lane, by the gfx950 permlane
+// swaps (rows 0<->1, 2<->3; lane halves)
+__device__ __forceinline__ float md_add_xor16(float x) {
+  auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+  return __uint_as_float(p[0]) + __uint_as_float(p[1]);
+}
+__device__ __forceinline__ float md_add_xor32(float x) {
+  auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+  return __uint_as_float(p[0]) + __uint_as_float(p[1]);
+}
+
+// The sum over the 8 lanes sharing l%8 (l, l^8, l^16, ... l^56), the same
+// on each: ((x + x^8) + (.)^16) + (.)^32, so the bits match a shuffle
+// butterfly of the same order.
+__device__ __forceinline__ float md_sum_same_j(float x) {
+  x += md_dpp<MD_DPP_ROR8>(x);  // within a row, lane (l+8) mod 16 is l^8
+  x = md_add_xor16(x);
+  return md_add_xor32(x);
+}
+
+// The sum over the 8 lanes sharing l/8, valid on the lanes with l%8 in
+// 0-3 (HIGH false) or 4-7 (HIGH true): the butterfly's ((x + x^1) + (.)^2)
+// then + (.)^4, whose partner l^4 is lane l+4 (l%8 < 4) or l-4 (l%8 >= 4).
+template <bool HIGH>
+__device__ __forceinline__ float md_sum_same_i(float x) {
+  x += md_dpp<MD_DPP_XOR1>(x);
+  x += md_dpp<MD_DPP_XOR2>(x);
+  return x + md_dpp<HIGH ? MD_DPP_ROR12 : MD_DPP_ROR4>(x);
+}
+
+// gap between boxes [ilo, ihi] and [jlo, jhi] on one axis, in frame units
+// (exact integer differences for boxes within 32 nm of each other)
 __device__ __forceinline__ float md_box_gap(int ilo, int ihi, int jlo, int jhi) {
   const int a = (int)((unsigned)jlo - (unsigned)ihi), b = (int)((unsigned)ilo - (unsigned)jhi);
-  return (float)max(0, max(a, b)) * MD_FRAME_UNIT;
+  return (float)max(0, max(a, b));
 }
 
 // One segment: consecutive list entries of super-cluster S against their j
@@ -910,8 +980,11 @@ __device__ __forceinline__ float md_box_gap(int ilo, int ihi, int jlo, int jhi) 
 template <bool ENERGY>
 __device__ __forceinline__ void md_pair_segment(const md_pair_args &A, int seg, int lane, int il, int jl,
                                                 int64_t &efix, int &count, int &blocks) {
-  int4 sg = A.segments[seg];
-  int S = sg.x, eb = sg.y, ee = sg.z;
+  const int4 sg = md_uniform_xyz(A.segments + seg);
+  // wave-uniform by construction; readfirstlane tells the compiler, so the
+  // entry loop runs on scalar registers and its loads are scalar loads
+  const int S = __builtin_amdgcn_readfirstlane(sg.x), eb = __builtin_amdgcn_readfirstlane(sg.y),
+            ee = __builtin_amdgcn_readfirstlane(sg.z);
   // i data: lane's atom in each of S's 8 clusters (8 clusters x 8 atoms = 64
   // slots; lane il*8+jl reads slot S*64 + q*8 + il)
   int xi[8], yi[8], zi[8];
@@ -925,9 +998,9 @@ __device__ __forceinline__ void md_pair_segment(const md_pair_args &A, int seg, 
     xi[q] = p.x;
     yi[q] = p.y;
     zi[q] = p.z;
-    qi[q] = __int_as_float(p.w);
-    si[q] = lj.x;
-    ei[q] = lj.y;
+    qi[q] = __int_as_float(p.w) * MD_PAIR_Q_SCALE;
+    si[q] = lj.x * MD_PAIR_S_SCALE;
+    ei[q] = lj.y * MD_PAIR_E_SCALE;
     fx[q] = fy[q] = fz[q] = 0.0f;
   }
   float en = 0.0f;
@@ -941,9 +1014,9 @@ __device__ __forceinline__ void md_pair_segment(const md_pair_args &A, int seg, 
     ihi = A.cluster_hi[S * 8 + lane];
   }
   for (int e = eb; e < ee; ++e) {
-    const int J = A.entry_j[e];
-    const int meta = A.entry_imask[e];
-    const int4 sh = A.entry_shift[e];
+    const int J = md_uniform(A.entry_j + e);
+    const int meta = md_uniform(A.entry_imask + e);
+    const int4 sh = md_uniform_xyz(A.entry_shift + e);
     // Blocks whose current boxes lie more than the cutoff apart (by a 1e-5
     // relative margin over the f32 distance test) have no pair inside it and
     // would add exactly zero: drop them before any pair arithmetic.  Lane q
@@ -952,14 +1025,14 @@ __device__ __forceinline__ void md_pair_segment(const md_pair_args &A, int seg, 
     // its pairs being masked.
     int active = 0;
     {
-      const int4 jlo = A.cluster_lo[J], jhi = A.cluster_hi[J];
+      const int4 jlo = md_uniform_xyz(A.cluster_lo + J), jhi = md_uniform_xyz(A.cluster_hi + J);
       const float dx = md_box_gap(ilo.x, ihi.x, (int)((unsigned)jlo.x + (unsigned)sh.x),
                                   (int)((unsigned)jhi.x + (unsigned)sh.x));
       const float dy = md_box_gap(ilo.y, ihi.y, (int)((unsigned)jlo.y + (unsigned)sh.y),
                                   (int)((unsigned)jhi.y + (unsigned)sh.y));
       const float dz = md_box_gap(ilo.z, ihi.z, (int)((unsigned)jlo.z + (unsigned)sh.z),
                                   (int)((unsigned)jhi.z + (unsigned)sh.z));
-      const bool near = lane < 8 && fmaf(dx, dx, fmaf(dy, dy, dz * dz)) < A.prune2;
+      const bool near = lane < 8 && fmaf(dx, dx, fmaf(dy, dy, dz * dz)) < A.prune2;  // frame units^2
       active = meta & (int)(__ballot(near) & 0xffu);
     }
     if (ENERGY) blocks += __popc(active);
@@ -967,7 +1040,8 @@ __device__ __forceinline__ void md_pair_segment(const md_pair_args &A, int seg, 
     if (active) {
       const int4 pj = A.slot_xyzq[J * MD_CLUSTER + jl];
       const float2 lj = A.slot_lj[J * MD_CLUSTER + jl];
-      const float qj = __int_as_float(pj.w);
+      const float qj = __int_as_float(pj.w) * MD_PAIR_Q_SCALE, sj = lj.x * MD_PAIR_S_SCALE,
+                  ej = lj.y * MD_PAIR_E_SCALE;
       // wrapping (unsigned) integer arithmetic: exact for every real pair; the
       // padding slots far out may wrap, and their pairs are masked
       unsigned xj = (unsigned)pj.x + (unsigned)sh.x, yj = (unsigned)pj.y + (unsigned)sh.y,
@@ -976,15 +1050,18 @@ __device__ __forceinline__ void md_pair_segment(const md_pair_args &A, int seg, 
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         if ((active >> q) & 1) {
-          uint64_t m = ex[q];
-          bool allowed = (m >> lane) & 1;
+          // bit l of the mask allows lane l's pair: with the in-cutoff ballot
+          // it is the block's lane mask of counted pairs
+          const uint64_t allowed = md_uniform(ex + q);
           float dx = (float)(int)(xj - (unsigned)xi[q]), dy = (float)(int)(yj - (unsigned)yi[q]),
                 dz = (float)(int)(zj - (unsigned)zi[q]);
-          float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz)) * MD_FRAME_UNIT2;
+          float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+          const uint64_t inc = allowed & __ballot(r2 < A.cutoff2);
           // a block whose 64 pairs all lie beyond the cutoff (or are masked)
           // adds exactly zero: skip its arithmetic, wave-uniformly
-          if (__ballot(allowed && r2 < A.cutoff2) == 0) continue;
-          float fs = md_pair<ENERGY>(r2, qi[q], si[q], ei[q], qj, lj.x, lj.y, allowed, A, en, count);
+          if (inc == 0) continue;
+          float fs = md_pair<ENERGY>(r2, qi[q], si[q], ei[q], qj, sj, ej,
+                                     __builtin_amdgcn_inverse_ballot_w64(inc), A, en, count);
           fx[q] = fmaf(-fs, dx, fx[q]);
           fy[q] = fmaf(-fs, dy, fy[q]);
           fz[q] = fmaf(-fs, dz, fz[q]);
@@ -994,26 +1071,27 @@ __device__ __forceinline__ void md_pair_segment(const md_pair_args &A, int seg, 
         }
       }
       // j force: sum the 8 lanes that share j (lane%8), fixed order
-#pragma unroll
-      for (int off = 8; off < 64; off <<= 1) {
-        gx += __shfl_xor(gx, off, 64);
-        gy += __shfl_xor(gy, off, 64);
-        gz += __shfl_xor(gz, off, 64);
-      }
+      gx = md_sum_same_j(gx);
+      gy = md_sum_same_j(gy);
+      gz = md_sum_same_j(gz);
     }
     // lanes 0-23 hold (x, y, z) of j atoms 0-7: one 96-byte store
     if (lane < 24) A.j_force[(size_t)e * 24 + lane] = lane < 8 ? gx : (lane < 16 ? gy : gz);
     if (e + 1 == unit_end) {  // wave-uniform: the end of a unit
-      // i forces: sum the 8 lanes that share i (same il), fixed order; every
-      // lane of the butterfly ends with the same sum, lane jl keeps q = jl's
+      // i forces: sum the 8 lanes that share i (same il), fixed order; lane
+      // jl keeps q = jl's sum
       float sx = 0.0f, sy = 0.0f, sz = 0.0f;
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        float x = fx[q], y = fy[q], z = fz[q];
-        for (int off = 1; off < 8; off <<= 1) {
-          x += __shfl_xor(x, off, 64);
-          y += __shfl_xor(y, off, 64);
-          z += __shfl_xor(z, off, 64);
+        float x, y, z;
+        if (q < 4) {
+          x = md_sum_same_i<false>(fx[q]);
+          y = md_sum_same_i<false>(fy[q]);
+          z = md_sum_same_i<false>(fz[q]);
+        } else {
+          x = md_sum_same_i<true>(fx[q]);
+          y = md_sum_same_i<true>(fy[q]);
+          z = md_sum_same_i<true>(fz[q]);
         }
         if (jl == q) {
           sx = x;
@@ -1026,7 +1104,7 @@ __device__ __forceinline__ void md_pair_segment(const md_pair_args &A, int seg, 
       ay += md_to_fixed(sy);
       az += md_to_fixed(sz);
       if (ENERGY) {
-        efix += md_to_fixed(en);
+        efix += md_to_fixed(en * MD_FRAME_UNIT);  // the energy carried 2^26
         en = 0.0f;
       }
       unit_end = min(unit_end + MD_SEGMENT, ee);
@@ -1380,11 +1458,11 @@ static md_nb_args nb_args(md_system *s, int seg_len) {
   P.blocks = &s->ctl->pair_blocks;
   P.nslots = s->nb.nslots;
   P.nclusters = s->nb.nclusters;
-  P.cutoff2 = s->nb.cutoff2;
-  P.prune2 = (float)((double)s->nb.cutoff2 * (1.0 + 1e-5));
-  P.alpha = (float)s->alpha;
-  P.beta = (float)(2.0 * s->alpha / std::sqrt(M_PI));
-  P.exp_scale = (float)(-s->alpha * s->alpha / std::log(2.0));
+  P.cutoff2 = std::ldexp(s->nb.cutoff2, 52);
+  P.prune2 = std::ldexp((float)((double)s->nb.cutoff2 * (1.0 + 1e-5)), 52);
+  P.half_alpha = std::ldexp(0.5f * (float)s->alpha, -26);
+  P.beta = std::ldexp((float)(2.0 * s->alpha / std::sqrt(M_PI)), -26);
+  P.exp_scale = std::ldexp((float)(-s->alpha * s->alpha / std::log(2.0)), -52);
   a.pos = s->pos;
   a.cell_rank = s->cell_rank;
   a.cell_count = s->cell_count;

@@ -217,59 +217,83 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+// One Stockham stage of radix R (a compile-time constant, so a butterfly's R
+// values stay in registers): output j of sub-transform k gathers the R
+// inputs j + q m, multiplies input q by the twiddle w^(k q n / (ns R)) (an
+// index below n, as k < ns and q < R), and takes their length-R DFT.
+template <int R>
+__device__ __forceinline__ void md_fft_stage(const float2 *in, float2 *out, int nlines, int n, int ns,
+                                             const float2 *tw, float sign) {
+  const int m = n / R;
+  const int twstride = n / (ns * R);
+  const int dftstride = n / R;
+  for (int idx = threadIdx.x; idx < nlines * m; idx += blockDim.x) {
+    const int line = idx / m, j = idx - line * m;
+    const int k = j % ns;
+    float2 v[R];
+    const float2 *src = in + line * n;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      float2 x = src[j + q * m];
+      if (q > 0 && k > 0) {
+        float2 w = tw[k * q * twstride];
+        w.y *= -sign;  // table holds exp(-2 pi i t/n); inverse conjugates
+        x = cmul(x, w);
+      }
+      v[q] = x;
+    }
+    float2 *dst = out + line * n + (j / ns) * ns * R + k;
+    if constexpr (R == 2) {
+      dst[0] = make_float2(v[0].x + v[1].x, v[0].y + v[1].y);
+      dst[ns] = make_float2(v[0].x - v[1].x, v[0].y - v[1].y);
+    } else if constexpr (R == 4) {
+      float2 s0 = make_float2(v[0].x + v[2].x, v[0].y + v[2].y);
+      float2 s1 = make_float2(v[0].x - v[2].x, v[0].y - v[2].y);
+      float2 s2 = make_float2(v[1].x + v[3].x, v[1].y + v[3].y);
+      float2 s3 = make_float2(v[1].x - v[3].x, v[1].y - v[3].y);
+      // multiply s3 by sign*i: forward (-i), inverse (+i)
+      float2 t3 = make_float2(-sign * s3.y, sign * s3.x);
+      dst[0] = make_float2(s0.x + s2.x, s0.y + s2.y);
+      dst[ns] = make_float2(s1.x + t3.x, s1.y + t3.y);
+      dst[2 * ns] = make_float2(s0.x - s2.x, s0.y - s2.y);
+      dst[3 * ns] = make_float2(s1.x - t3.x, s1.y - t3.y);
+    } else {
+#pragma unroll
+      for (int qo = 0; qo < R; ++qo) {
+        float2 acc = v[0];
+#pragma unroll
+        for (int q = 1; q < R; ++q) {
+          float2 w = tw[((qo * q) % R) * dftstride];
+          w.y *= -sign;
+          float2 p = cmul(v[q], w);
+          acc.x += p.x;
+          acc.y += p.y;
+        }
+        dst[qo * ns] = acc;
+      }
+    }
+  }
+}
+
 // `nlines` contiguous lines of length P.n in `a` (pitch P.n), transformed with
 // sign -1 (forward) or +1 (inverse); returns the buffer holding the result.
-__device__ float2 *md_lds_fft(float2 *a, float2 *b, int nlines, const md_fft_plan &P, const float2 *tw, float sign) {
-  int n = P.n;
+// Each stage dispatches on its radix to the specialised butterfly.
+__device__ __forceinline__ float2 *md_lds_fft(float2 *a, float2 *b, int nlines, const md_fft_plan &P,
+                                              const float2 *tw_all, float sign) {
+  const int n = P.n;
+  const float2 *tw = tw_all + P.tw;
   int ns = 1;
   float2 *in = a, *out = b;
   for (int s = 0; s < P.nstages; ++s) {
-    int r = P.radix[s];
-    int m = n / r;
-    int twstride = n / (ns * r);
-    int dftstride = n / r;
-    for (int idx = threadIdx.x; idx < nlines * m; idx += blockDim.x) {
-      int line = idx / m, j = idx - line * m;
-      int k = j % ns;
-      float2 v[16];
-      const float2 *src = in + line * n;
-      for (int q = 0; q < r; ++q) {
-        float2 x = src[j + q * m];
-        if (q > 0 && k > 0) {
-          float2 w = tw[P.tw + (k * q * twstride) % n];
-          w.y *= -sign;  // table holds exp(-2 pi i t/n); inverse conjugates
-          x = cmul(x, w);
-        }
-        v[q] = x;
-      }
-      float2 *dst = out + line * n + (j / ns) * ns * r + k;
-      if (r == 2) {
-        dst[0] = make_float2(v[0].x + v[1].x, v[0].y + v[1].y);
-        dst[ns] = make_float2(v[0].x - v[1].x, v[0].y - v[1].y);
-      } else if (r == 4) {
-        float2 s0 = make_float2(v[0].x + v[2].x, v[0].y + v[2].y);
-        float2 s1 = make_float2(v[0].x - v[2].x, v[0].y - v[2].y);
-        float2 s2 = make_float2(v[1].x + v[3].x, v[1].y + v[3].y);
-        float2 s3 = make_float2(v[1].x - v[3].x, v[1].y - v[3].y);
-        // multiply s3 by sign*i: forward (-i), inverse (+i)
-        float2 t3 = make_float2(-sign * s3.y, sign * s3.x);
-        dst[0] = make_float2(s0.x + s2.x, s0.y + s2.y);
-        dst[ns] = make_float2(s1.x + t3.x, s1.y + t3.y);
-        dst[2 * ns] = make_float2(s0.x - s2.x, s0.y - s2.y);
-        dst[3 * ns] = make_float2(s1.x - t3.x, s1.y - t3.y);
-      } else {
-        for (int qo = 0; qo < r; ++qo) {
-          float2 acc = v[0];
-          for (int q = 1; q < r; ++q) {
-            float2 w = tw[P.tw + ((qo * q) % r) * dftstride];
-            w.y *= -sign;
-            float2 p = cmul(v[q], w);
-            acc.x += p.x;
-            acc.y += p.y;
-          }
-          dst[qo * ns] = acc;
-        }
-      }
+    const int r = P.radix[s];
+    switch (r) {
+      case 2: md_fft_stage<2>(in, out, nlines, n, ns, tw, sign); break;
+      case 3: md_fft_stage<3>(in, out, nlines, n, ns, tw, sign); break;
+      case 4: md_fft_stage<4>(in, out, nlines, n, ns, tw, sign); break;
+      case 5: md_fft_stage<5>(in, out, nlines, n, ns, tw, sign); break;
+      case 7: md_fft_stage<7>(in, out, nlines, n, ns, tw, sign); break;
+      case 11: md_fft_stage<11>(in, out, nlines, n, ns, tw, sign); break;
+      default: md_fft_stage<13>(in, out, nlines, n, ns, tw, sign); break;
     }
     __syncthreads();
     float2 *t = in;

@@ -182,67 +182,72 @@ def _i(a: np.ndarray):
     return a.ctypes.data_as(_ip)
 
 
-def describe(system, box: Optional[Sequence] = None):
-    """Pack an OpenMMSystem into ``md_system_desc``.  Returns (desc, keepalive)."""
-    keep = []
-
-    def f64(x):
-        a = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
-        keep.append(a)
-        return _d(a)
-
-    def i32(x):
-        a = np.ascontiguousarray(np.asarray(x, dtype=np.int32))
-        keep.append(a)
-        return _i(a)
-
-    d = SystemDesc()
-    d.n_particles = system.particleCount
-    d.masses = f64(system.masses)
-    b = np.asarray(box if box is not None else system.boxVectors, dtype=np.float64).reshape(9)
-    for k in range(9):
-        d.box[k] = b[k]
-    d.n_constraints = system.constraintCount
-    if d.n_constraints:
-        d.constraint_p1, d.constraint_p2 = i32(system.constraintP1), i32(system.constraintP2)
-        d.constraint_distance = f64(system.constraintDistance)
+def describe_fields(system, box: Optional[Sequence] = None) -> dict:
+    """``md_system_desc``'s fields for an OpenMMSystem, by name: numbers, and
+    numpy arrays (float64 / int32) for the arrays.  The same record a Node
+    host hands the N-API binding (``node/engine.js``)."""
+    f64 = lambda x: np.ascontiguousarray(np.asarray(x, dtype=np.float64))  # noqa: E731
+    i32 = lambda x: np.ascontiguousarray(np.asarray(x, dtype=np.int32))  # noqa: E731
+    d = {"n_particles": system.particleCount, "masses": f64(system.masses),
+         "box": f64(np.asarray(box if box is not None else system.boxVectors, dtype=np.float64).reshape(9)),
+         "n_constraints": system.constraintCount}
+    if system.constraintCount:
+        d.update(constraint_p1=i32(system.constraintP1), constraint_p2=i32(system.constraintP2),
+                 constraint_distance=f64(system.constraintDistance))
     for f in system.forces:
         t = f.type
         if t == "HarmonicBondForce" and f.count:
-            d.n_bonds, d.bond_p1, d.bond_p2 = f.count, i32(f.p1), i32(f.p2)
-            d.bond_length, d.bond_k = f64(f.length), f64(f.k)
+            d.update(n_bonds=f.count, bond_p1=i32(f.p1), bond_p2=i32(f.p2), bond_length=f64(f.length),
+                     bond_k=f64(f.k))
         elif t == "HarmonicAngleForce" and f.count:
-            d.n_angles, d.angle_p1, d.angle_p2, d.angle_p3 = f.count, i32(f.p1), i32(f.p2), i32(f.p3)
-            d.angle_theta, d.angle_k = f64(f.angle), f64(f.k)
+            d.update(n_angles=f.count, angle_p1=i32(f.p1), angle_p2=i32(f.p2), angle_p3=i32(f.p3),
+                     angle_theta=f64(f.angle), angle_k=f64(f.k))
         elif t == "PeriodicTorsionForce" and f.count:
-            d.n_torsions = f.count
-            d.torsion_p1, d.torsion_p2, d.torsion_p3, d.torsion_p4 = (i32(f.p1), i32(f.p2), i32(f.p3),
-                                                                      i32(f.p4))
-            d.torsion_periodicity, d.torsion_phase, d.torsion_k = (i32(f.periodicity), f64(f.phase),
-                                                                   f64(f.k))
+            d.update(n_torsions=f.count, torsion_p1=i32(f.p1), torsion_p2=i32(f.p2), torsion_p3=i32(f.p3),
+                     torsion_p4=i32(f.p4), torsion_periodicity=i32(f.periodicity), torsion_phase=f64(f.phase),
+                     torsion_k=f64(f.k))
         elif t == "NonbondedForce":
             if f.method != 4:
                 raise MdError(1, f"NonbondedForce method {f.method}: this engine implements PME (4)")
-            d.has_nonbonded = 1
-            d.cutoff, d.ewald_tolerance, d.alpha = f.cutoff, f.ewaldTolerance, f.alpha
-            for k in range(3):
-                d.grid[k] = int(f.grid[k])
-            d.use_dispersion_correction = int(bool(f.useDispersionCorrection))
-            d.charge, d.sigma, d.epsilon = f64(f.charge), f64(f.sigma), f64(f.epsilon)
-            d.n_exceptions = f.exceptionCount
+            if f.useSwitchingFunction:
+                raise MdError(1, "NonbondedForce with a switching function: this engine implements the plain "
+                                 "cutoff of the reference pair term")
+            d.update(has_nonbonded=1, cutoff=float(f.cutoff), ewald_tolerance=float(f.ewaldTolerance),
+                     alpha=float(f.alpha), grid=i32([int(g) for g in f.grid]),
+                     use_dispersion_correction=int(bool(f.useDispersionCorrection)),
+                     charge=f64(f.charge), sigma=f64(f.sigma), epsilon=f64(f.epsilon),
+                     n_exceptions=f.exceptionCount)
             if f.exceptionCount:
-                d.exception_p1, d.exception_p2 = i32(f.exceptionP1), i32(f.exceptionP2)
-                d.exception_chargeprod = f64(f.exceptionChargeProd)
-                d.exception_sigma, d.exception_epsilon = f64(f.exceptionSigma), f64(f.exceptionEpsilon)
+                d.update(exception_p1=i32(f.exceptionP1), exception_p2=i32(f.exceptionP2),
+                         exception_chargeprod=f64(f.exceptionChargeProd), exception_sigma=f64(f.exceptionSigma),
+                         exception_epsilon=f64(f.exceptionEpsilon))
         elif t == "PositionRestraint" and f.count:
-            d.n_restraints, d.restraint_particle, d.restraint_k = f.count, i32(f.particle), f64(f.k)
-            d.restraint_x0, d.restraint_y0, d.restraint_z0 = f64(f.x0), f64(f.y0), f64(f.z0)
+            d.update(n_restraints=f.count, restraint_particle=i32(f.particle), restraint_k=f64(f.k),
+                     restraint_x0=f64(f.x0), restraint_y0=f64(f.y0), restraint_z0=f64(f.z0))
         elif t == "CMMotionRemover":
-            d.cmm_frequency = int(f.frequency)
+            d["cmm_frequency"] = int(f.frequency)
         elif t == "MonteCarloBarostat":
-            d.has_barostat = 1
-            d.barostat_pressure, d.barostat_temperature = f.defaultPressure, f.defaultTemperature
-            d.barostat_frequency, d.barostat_seed = int(f.frequency), int(f.randomSeed) & 0xFFFFFFFF
+            d.update(has_barostat=1, barostat_pressure=float(f.defaultPressure),
+                     barostat_temperature=float(f.defaultTemperature), barostat_frequency=int(f.frequency),
+                     barostat_seed=int(f.randomSeed) & 0xFFFFFFFF)
+    return d
+
+
+def describe(system, box: Optional[Sequence] = None):
+    """Pack an OpenMMSystem into ``md_system_desc``.  Returns (desc, keepalive)."""
+    fields = describe_fields(system, box)
+    keep = []
+    d = SystemDesc()
+    for name, value in fields.items():
+        if name in ("box", "grid"):
+            arr = getattr(d, name)
+            for k in range(len(arr)):
+                arr[k] = value[k]
+        elif isinstance(value, np.ndarray):
+            keep.append(value)
+            setattr(d, name, _d(value) if value.dtype == np.float64 else _i(value))
+        else:
+            setattr(d, name, value)
     return d, keep
 
 

@@ -7,7 +7,9 @@ no network and no force-field files:
   ~111 degree angles, random torsions with n in {1,2,3} and k in [0,10]
   kJ/mol), 0-3 hydrogens per heavy atom bonded at 0.109 nm and constrained
   (H-bond clusters of 1-3 constraints, 2-4 atoms), sigma in [0.30,0.40] nm,
-  epsilon in [0.2,0.9] kJ/mol, charges N(0, 0.3) shifted to neutral;
+  epsilon in [0.2,0.9] kJ/mol, charges drawn from N(0, 0.3) -- hydrogens
+  positive as in AMBER (the draw's magnitude, at most 0.45 e) -- and made
+  neutral over the heavy atoms;
   1-2 and 1-3 pairs excluded, 1-4 pairs scaled 1/1.2 (Coulomb) and 1/2 (LJ);
 * an optional 24-atom ligand built the same way (GAFF-like random parameters);
 * TIP3P water (O-H 0.09572 nm, H-H 0.15139 nm, q -0.834/+0.417,
@@ -30,6 +32,7 @@ from typing import List, Optional, Tuple
 import numpy as np
 
 from .state import OpenMMState
+from .units import ewaldAlpha, pmeMinimumGridSize
 from .system import (CMMotionRemover, HarmonicAngleForce, HarmonicBondForce, MonteCarloBarostat,
                      NonbondedForce, OpenMMSystem, PeriodicTorsionForce, PositionRestraintForce,
                      boxVectorsForShape)
@@ -301,10 +304,18 @@ def _chain(top: _Topology, rng, n_atoms: int, center, radius: float, wall: float
     for a, b, c, d in zip(heavy[:-3], heavy[1:-2], heavy[2:-1], heavy[3:]):
         n = int(rng.integers(1, 4))
         top.tors.append((a, b, c, d, n, float(rng.uniform(0, math.pi)), float(rng.uniform(0, 10))))
-    # charges: N(0, 0.3), neutralised over the chain
+    # charges: N(0, 0.3) draws, with AMBER's sign convention for hydrogens --
+    # a hydrogen takes the magnitude of its draw, capped at 0.45 e (AMBER's
+    # hydrogens are all positive, at most ~0.45 e) -- neutralised over the
+    # heavy atoms.  A negatively charged hydrogen has nothing to keep a water
+    # hydrogen (no LJ) away but the water oxygen's repulsion; such pairs sat
+    # 0.11 nm apart in stiff wells that 2 fs steps could not integrate.
     idx = np.arange(start, top.n)
     q = rng.normal(0, 0.3, idx.shape[0])
-    q -= q.mean()
+    is_h = np.array([top.mass[i] < 1.5 for i in idx])
+    q[is_h] = np.minimum(np.abs(q[is_h]), 0.45)
+    heavy_k = ~is_h
+    q[heavy_k] -= q.sum() / max(1, int(heavy_k.sum()))
     for k, i in enumerate(idx):
         top.q[i] = float(q[k])
     top.heavy.extend(heavy)
@@ -455,23 +466,24 @@ def build(n_chain: int, n_waters: int, n_ions: int = 0, shape: str = "cube", see
 def config(index: int, seed: Optional[int] = None) -> Built:
     """The benchmark configurations, sized to their stated atom counts.
 
-    0: 2,373 atoms: a 304-atom chain + 689 TIP3P + 2 ions, cube (~2.9 nm)
+    0: 2,373 atoms: a 304-atom chain + 689 TIP3P + 2 ions, cube (~2.9 nm),
+       PME mesh 32^3
     1: 26,749 atoms: a 2,600-atom chain + 24-atom ligand + 8 ions + 8,039
        TIP3P, truncated octahedron at 1.0 g/cm^3 (d ~ 7.0 nm; the stated
-       6.5 nm does not hold the stated water at that density)
+       6.5 nm does not hold the stated water at that density), mesh 64^3
     2: ~100k atoms: a 1,000-atom solute + 33,000 TIP3P, rhombic dodecahedron,
-       barostat and heavy-atom restraints (k = 1000)
-    4: ~250k atoms: a 20,000-atom chain + 76,667 TIP3P, cube
+       barostat and heavy-atom restraints (k = 1000), mesh 96^3
+    4: ~250k atoms: a 20,000-atom chain + 76,667 TIP3P, cube, mesh 128^3
     """
     s = 0x20261202 + index if seed is None else seed
     if index == 0:
-        return build(304, 689, 2, "cube", s)
+        return build(304, 689, 2, "cube", s, grid=(32, 32, 32))
     if index == 1:
-        return build(2600, 8039, 8, "octahedron", s, n_ligand=24)
+        return build(2600, 8039, 8, "octahedron", s, n_ligand=24, grid=(64, 64, 64))
     if index == 2:
-        return build(1000, 33000, 0, "dodecahedron", s, restraint_k=1000.0, barostat=True)
+        return build(1000, 33000, 0, "dodecahedron", s, restraint_k=1000.0, barostat=True, grid=(96, 96, 96))
     if index == 4:
-        return build(20000, 76667, 0, "cube", s)
+        return build(20000, 76667, 0, "cube", s, grid=(128, 128, 128))
     raise ValueError(f"no configuration {index} (the 64-replica ensemble is `ensemble()`)")
 
 
@@ -497,10 +509,30 @@ def ensemble_plan(count: int = 64, seed: Optional[int] = None, low: int = 2000, 
         chain = max(40, int(round(0.1 * n)))
         ions = 2 * max(1, n // 10000)
         waters = max(10, (n - chain - ions) // 3)
-        plan.append(dict(n_chain=chain, n_waters=waters, n_ions=ions,
-                         shape="cube" if k % 2 == 0 else "octahedron",
-                         seed=int(seeds[k]), temperature=float(temps[k])))
+        shape = "cube" if k % 2 == 0 else "octahedron"
+        plan.append(dict(n_chain=chain, n_waters=waters, n_ions=ions, shape=shape,
+                         seed=int(seeds[k]), temperature=float(temps[k]),
+                         grid=ensemble_mesh(chain, waters, ions, shape)))
     return plan
+
+
+def ensemble_mesh(n_chain: int, n_waters: int, n_ions: int, shape: str) -> Tuple[int, int, int]:
+    """A replica's PME mesh: n^3 with n the smallest multiple of 8 at or above
+    the Ewald minimum for the cell's longest axis at tolerance 5e-4 (1.0 nm
+    cutoff), kept within 32-64 -- 32^3 for the smallest replicas, 56^3 for the
+    largest (finer than the minimum, never coarser)."""
+    volume = n_waters / WATER_DENSITY + 0.0125 * n_chain + n_ions * 0.03
+    width = (volume / SHAPE_VOLUME[shape]) ** (1.0 / 3.0)
+    alpha = ewaldAlpha(1.0, 5e-4)
+    n = pmeMinimumGridSize(alpha, width, 5e-4)
+    n = min(64, max(32, 8 * math.ceil(n / 8)))
+    return (n, n, n)
+
+
+def build_planned(p: dict) -> Built:
+    """One replica of ``ensemble_plan`` (module level, so worker processes
+    can build replicas in parallel)."""
+    return build(p["n_chain"], p["n_waters"], p["n_ions"], p["shape"], p["seed"], grid=p["grid"])
 
 
 def ensemble(count: int = 64, seed: Optional[int] = None, low: int = 2000, high: int = 30000):
@@ -508,6 +540,6 @@ def ensemble(count: int = 64, seed: Optional[int] = None, low: int = 2000, high:
     ``ensemble_plan``), each with its own seed and temperature."""
     out = []
     for p in ensemble_plan(count, seed, low, high):
-        b = build(p["n_chain"], p["n_waters"], p["n_ions"], p["shape"], p["seed"])
+        b = build(p["n_chain"], p["n_waters"], p["n_ions"], p["shape"], p["seed"], grid=p["grid"])
         out.append(Replica(b, p["seed"], p["temperature"]))
     return out

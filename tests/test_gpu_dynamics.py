@@ -182,7 +182,7 @@ def test_stale_list_evaluates_every_in_cutoff_pair(prepared):
         e.run(1)
         after = e._native.list_flags()
         # between a quarter and half the skin: no rebuild is pending either
-        if after["rebuilds"] == before["rebuilds"] and 0.25 * skin < after["worst_displacement"] < 0.5 * skin:
+        if after["rebuilds"] == before["rebuilds"] and 0.25 * skin < after["worstDisplacement"] < 0.5 * skin:
             stale = after
             break
     assert stale is not None, "the list never became stale enough"
@@ -197,7 +197,7 @@ def test_stale_list_evaluates_every_in_cutoff_pair(prepared):
     tree = cKDTree(np.mod(x.reshape(-1, 3), L), boxsize=L)
     rc = b.system.force("NonbondedForce").cutoff
     edge = len(tree.query_pairs(rc + 2e-6)) - len(tree.query_pairs(rc - 2e-6))
-    print("worst displacement", stale["worst_displacement"], "stale census", pairs_stale, "fresh", pairs_fresh,
+    print("worst displacement", stale["worstDisplacement"], "stale census", pairs_stale, "fresh", pairs_fresh,
           "oracle", d.pairs_geometric, "edge pairs", edge)
     assert abs(pairs_stale - d.pairs_geometric) <= edge
     assert abs(pairs_stale - pairs_fresh) <= edge

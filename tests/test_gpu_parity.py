@@ -106,10 +106,17 @@ def test_fixture_energies_match_openmm(fixture10):
 
 # ---- synthetic systems against the oracle ------------------------------------------
 
+def system_mesh(system, o):
+    """The mesh the System names (the configurations' stated meshes), else
+    the tolerance-derived default."""
+    g = tuple(int(x) for x in system.force("NonbondedForce").grid)
+    return g if all(g) else o.default_grid(system.boxVectors)
+
+
 def _compare(built, mesh=None):
     system, state = built.system, built.state
     o = O.OracleSystem(system)
-    grid = mesh or o.default_grid(system.boxVectors)
+    grid = mesh or system_mesh(system, o)
     s = native.NativeSystem(system, mesh=grid)
     s.set_positions(state.positions)
     total, groups, forces = s.evaluate()
@@ -119,7 +126,9 @@ def _compare(built, mesh=None):
 
 
 def test_config0_forces_and_groups(c0):
+    assert tuple(c0.system.force("NonbondedForce").grid) == (32, 32, 32)
     s, (total, groups, forces), (otot, og, of), err = _compare(c0)
+    assert tuple(s.mesh()) == (32, 32, 32)
     print("config 0 force rel RMS", rel_rms(forces, of), "edge-aware", err, "groups", groups, og)
     assert err < FORCE_RMS
     for k in range(6):
@@ -129,7 +138,9 @@ def test_config0_forces_and_groups(c0):
 
 
 def test_config1_triclinic_forces_and_groups(c1):
+    assert tuple(c1.system.force("NonbondedForce").grid) == (64, 64, 64)
     s, (total, groups, forces), (otot, og, of), err = _compare(c1)
+    assert tuple(s.mesh()) == (64, 64, 64)
     print("config 1 force rel RMS", rel_rms(forces, of), "edge-aware", err, "groups", groups, og)
     assert err < FORCE_RMS
     for k in range(6):
