@@ -72,7 +72,8 @@ def device_synchronize():
 
 def _build_one(plan):
     from mdcore import synthetic
-    b = synthetic.build(plan["n_chain"], plan["n_waters"], plan["n_ions"], plan["shape"], plan["seed"])
+    b = synthetic.build(plan["n_chain"], plan["n_waters"], plan["n_ions"], plan["shape"], plan["seed"],
+                        grid=plan["grid"])
     return synthetic.Replica(b, plan["seed"], plan["temperature"])
 
 
@@ -104,6 +105,31 @@ def oracle_rate(system, positions, velocities, budget_s, temperature=300.0, seed
             break
     el = time.perf_counter() - t0
     return system.particleCount * steps / el, steps, el
+
+
+def _oracle_rate_job(job):
+    """One worker's CPU baseline: the oracle on one replica for ~budget s."""
+    system, positions, velocities, budget, temperature, seed = job
+    rate, steps, el = oracle_rate(system, positions, velocities, budget, temperature, seed)
+    return rate, steps, el, system.particleCount
+
+
+def host_cpu():
+    """(logical CPUs usable here, the CPU model name)."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return usable, model
 
 
 def force_error(engine, system):
@@ -218,8 +244,10 @@ def run_ensemble(args):
 
     # ---- plain: replicas one after another through the single-system path --
     plain_steps = args.plain_steps
-    for e in engines:  # capture each replica's stride graph outside the timed region
+    for e in engines:  # capture each replica's graphs outside the timed region
         e.run(50)
+        if plain_steps % 50:
+            e.run(plain_steps % 50)
     device_synchronize()
     t0 = time.perf_counter()
     for e in engines:
@@ -233,7 +261,14 @@ def run_ensemble(args):
     # ---- batched, plain SHAKE ----------------------------------------------------
     from mdcore import native
     batch = HipBatch(engines)
+    # the pair kernel's schedule for this batch, picked by timing (changes no bits)
+    tune = batch.autotune(args.tune_steps) if args.tune_steps > 0 else None
+    if tune:
+        log(f"autotuned pair schedule: {tune['segment_entries']} entries per unit, {tune['pair_waves']} waves "
+            f"({tune['ms_per_step']:.3f} ms/step phase by phase) vs the default {tune['default_segment_entries']}, "
+            f"{tune['default_pair_waves']} ({tune['default_ms_per_step']:.3f} ms/step)")
     batch.run(max(args.warmup, 50))
+    batch.run(args.steps)  # captures the graph of the timed run's length outside the timed region
     device_synchronize()
     counts0 = [native.shake_counts(e._native) for e in engines]
     t0 = time.perf_counter()
@@ -264,6 +299,7 @@ def run_ensemble(args):
 
     # ---- batched + learned: the measured configuration ---------------------------
     batch.run(max(args.warmup, 50))
+    batch.run(args.steps)  # the cluster kernel changed: capture the timed length again
     device_synchronize()
     counts0 = [native.shake_counts(e._native) for e in engines]
     rebuilds0 = [e._native.list_flags()["rebuilds"] for e in engines]
@@ -303,10 +339,30 @@ def run_ensemble(args):
         pairs += p
     log("batched phases (ms): " + ", ".join(f"{k} {v:.4f}" for k, v in phases.items()))
     integ_gbs = 124.0 * total_atoms / (phases["integrate"] * 1e-3) / 1e9 if phases["integrate"] > 0 else 0.0
+    meshes = [tuple(e.mesh) for e in engines]
+    points = float(sum(m[0] * m[1] * m[2] for m in meshes))
+
+    def hbm(phase, nbytes):
+        ms = phases.get(phase, 0.0)
+        gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        return {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                "bytes": nbytes, "ms": ms}
+
+    # algorithmic bytes per launch, per unit of work: spreading 28 B/atom + 8 B
+    # per grid point; the five transform passes (C2C forward + inverse, 3 axes
+    # each) 96 B/point plus the convolution's 16; gathering 28 B/atom + 8 B per
+    # point + 24 B/atom of force
+    pme_roofline = {
+        "pme_spread": hbm("pme_spread", 28.0 * total_atoms + 8.0 * points),
+        "pme_fft_convolve": hbm("pme_fft_convolve", (96.0 + 16.0) * points),
+        "pme_gather": hbm("pme_gather", (28.0 + 24.0) * total_atoms + 8.0 * points),
+        "grid_points": points,
+    }
 
     result = {
         "metric": METRIC,
-        "value": batch_nsday,
+        # a run in which any replica's constraints stopped holding does not stand
+        "value": batch_nsday if diverged == 0 else None,
         "unit": "ns/day",
         "n_gpus": 1,
         "steps": args.steps,
@@ -320,9 +376,10 @@ def run_ensemble(args):
                 "at 10/ps friction, then warmed up at 2 fs",
         "config": {
             "workload": f"batched ensemble + learned SHAKE warm start: {len(engines)} replicas, {total_atoms} atoms "
-                        f"(log-uniform {min(atoms)}-{max(atoms)}), cubic/octahedral cells, 290-320 K, PME 1.0 nm, "
-                        "LFMiddle, dt 2 fs, SHAKE + SETTLE; one launch per phase for all replicas, 50-step "
-                        "HIP-graph strides",
+                        f"(log-uniform {min(atoms)}-{max(atoms)}), cubic/octahedral cells, 290-320 K, PME 1.0 nm "
+                        "with 32^3-64^3 meshes by size, LFMiddle, dt 2 fs, SHAKE + SETTLE; one launch per phase "
+                        "for all replicas, steps replayed from captured HIP graphs (50-step strides, shorter runs "
+                        "their own length)",
             "replicas": len(engines),
             "atoms": total_atoms,
             "plain_ns_day": plain_nsday,
@@ -341,6 +398,7 @@ def run_ensemble(args):
             "worst_constraint_rel_deviation": worst_constraint,
             "diverged_replicas": diverged,
             "steps_per_list_rebuild": steps_per_rebuild,
+            "autotune": tune,
             "force_rel_rms_vs_oracle": errors,
             "in_cutoff_pairs": int(pairs),
             "arithmetic": "fp32 with Q32.32 int64 accumulation",
@@ -352,21 +410,48 @@ def run_ensemble(args):
         "phases_ms": phases,
         "integrator_roofline": {"bound": "hbm", "achieved": integ_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": integ_gbs / HBM_PEAK_GBS, "bytes_per_atom": 124},
+        "pme_roofline": pme_roofline,
     }
+    result["config"]["meshes"] = {f"{n}^3": sum(1 for m in meshes if m == (n, n, n))
+                                  for n in sorted({m[0] for m in meshes})}
+    if diverged:
+        result["error"] = f"{diverged} replica(s) no longer hold their constraints: the figure does not stand"
     if not args.no_cpu_baseline:
-        k = int(order[0])
-        log(f"timing the CPU oracle on the {atoms[k]}-atom replica for ~{args.cpu_seconds:.0f} s")
-        rate, steps, el = oracle_rate(reps[k].built.system, engines[k].positions(), engines[k].velocities(),
-                                      args.cpu_seconds, reps[k].temperature, reps[k].seed)
-        # the whole ensemble, one replica after another, at that atom-step rate
-        cpu_nsday = len(engines) * DT_PS * 1e-3 * rate / total_atoms * 86400.0
+        # The reference CPU path is single-threaded; the ensemble's replicas are
+        # independent, so the host's cores run one replica each: `workers`
+        # oracle processes on replicas spread over the size range, each timed
+        # for ~cpu_seconds, their atom-step rates summed.  One core alone is
+        # reported beside it.
+        usable, model = host_cpu()
+        workers = max(1, min(args.cpu_workers, usable))
+        picks = [int(order[round(i * (len(order) - 1) / max(1, workers - 1))]) for i in range(workers)]
+        log(f"timing the CPU oracle: {workers} processes on {usable} usable cores ({model}), "
+            f"~{args.cpu_seconds:.0f} s each")
+        jobs = [(reps[k].built.system, engines[k].positions(), engines[k].velocities(), args.cpu_seconds,
+                 reps[k].temperature, reps[k].seed) for k in picks]
+        if workers > 1:
+            import multiprocessing as mp
+            with mp.get_context("spawn").Pool(workers) as pool:
+                timed = pool.map(_oracle_rate_job, jobs, chunksize=1)
+        else:
+            timed = [_oracle_rate_job(j) for j in jobs]
+        all_rate = float(sum(r for r, _, _, _ in timed))
+        one_rate = float(timed[0][0])  # the smallest replica on one core
+        # the whole ensemble advancing one step costs total_atoms atom-steps
+        cpu_nsday = len(engines) * DT_PS * 1e-3 * all_rate / total_atoms * 86400.0
+        one_nsday = len(engines) * DT_PS * 1e-3 * one_rate / total_atoms * 86400.0
         result["cpu_baseline"] = {
-            "value": cpu_nsday, "unit": "ns/day", "cores": 1, "kind": "port",
-            "sample": f"{steps} LFMiddle steps of the {atoms[k]}-atom replica by the f64 C oracle on one host "
-                      f"core ({el:.1f} s, {rate:.0f} atom-steps/s), scaled to the whole ensemble run one "
-                      "replica after another at that atom-step rate",
-            "atom_steps_per_s": rate}
-        log(f"cpu baseline {cpu_nsday:.5f} ns/day aggregate")
+            "value": cpu_nsday, "unit": "ns/day", "cores": workers, "kind": "port",
+            "sample": f"the f64 C oracle (LFMiddle with SETTLE/SHAKE and the thermostat) in {workers} processes, "
+                      f"one replica each ({min(a for _, _, _, a in timed)}-{max(a for _, _, _, a in timed)} atoms), "
+                      f"~{args.cpu_seconds:.0f} s each; their atom-step rates summed and scaled to the whole "
+                      "ensemble",
+            "atom_steps_per_s": all_rate, "host_cpus": usable, "cpu_model": model,
+            "single_core": {"value": one_nsday, "unit": "ns/day", "atom_steps_per_s": one_rate,
+                            "replica_atoms": timed[0][3]},
+            "per_process": [{"atoms": a, "steps": st, "seconds": el, "atom_steps_per_s": r}
+                            for r, st, el, a in timed]}
+        log(f"cpu baseline {cpu_nsday:.4f} ns/day aggregate on {workers} cores, {one_nsday:.5f} on one")
     batch.destroy()
     for e in engines:
         native.NativeWarmStart.detach(e._native)
@@ -476,6 +561,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-workers", type=int, default=16,
+                    help="oracle processes for the CPU baseline (at most the usable cores)")
     ap.add_argument("--minimise-iterations", type=int, default=60)
     ap.add_argument("--equilibrate-steps", type=int, default=400)
     ap.add_argument("--equilibrate-dt", type=float, default=0.0005, help="ps")
@@ -486,6 +573,8 @@ def main():
     ap.add_argument("--learning-rate", type=float, default=1e-3)
     ap.add_argument("--ws-capacity", type=int, default=4_000_000)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--tune-steps", type=int, default=5,
+                    help="steps timed per pair-kernel schedule by the autotuner (0: no tuning)")
     ap.add_argument("--verlet-skin", type=float, default=None,
                     help="list skin in nm (default: the engine's, 0.15)")
     ap.add_argument("--max-batch", type=int, default=0,
@@ -506,7 +595,17 @@ def main():
     if args.traffic_probe > 0:
         run_traffic_probe(args)
         return
-    result = run_ensemble(args) if args.workload == "ensemble" else run_single(args)
+    from mdcore import native
+    try:
+        result = run_ensemble(args) if args.workload == "ensemble" else run_single(args)
+    except native.MdError as err:
+        # a replica diverged, overflowed its list or hit another engine error:
+        # no figure stands, and the line says why
+        print(json.dumps({"metric": METRIC, "value": None, "unit": "ns/day", "n_gpus": 1, "steps": args.steps,
+                          "warmup": args.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                          "dtype": "fp32", "data": "synthetic", "config": {"workload": args.workload},
+                          "error": str(err)}), flush=True)
+        sys.exit(1)
     if result is not None:
         print(json.dumps(result), flush=True)
 

@@ -121,6 +121,25 @@ typedef struct {
 
 typedef struct md_system md_system;
 
+/* Streaming System / State ingestion, host only (no device is touched).
+ * parseSystem (src/engine/system.ts:521-597) and parseState
+ * (src/engine/state.ts:60-152) as one pass over the XML text with no element
+ * tree, for systems of hundreds of thousands of atoms.  The same forces are
+ * accepted and refused as upstream (MD_ERR_UNSUPPORTED_FORCE with the
+ * reason; malformed documents MD_ERR_INVALID_ARGUMENT).  The description's
+ * arrays point into the parsed object and live until it is freed. */
+typedef struct md_xml_system md_xml_system;
+typedef struct md_xml_state md_xml_state;
+md_err md_xml_parse_system(const char *text, int64_t length, md_xml_system **out);
+md_err md_xml_system_describe(const md_xml_system *system, md_system_desc *desc);
+void md_xml_system_free(md_xml_system *system);
+md_err md_xml_parse_state(const char *text, int64_t length, md_xml_state **out);
+/* positions (and velocities, NULL when the State has none) xyz-interleaved
+ * nm and nm/ps; box the row vectors; step and time as written */
+md_err md_xml_state_get(const md_xml_state *state, int32_t *particles, const double **positions,
+                        const double **velocities, double box[9], int64_t *step, double *time);
+void md_xml_state_free(md_xml_state *state);
+
 /* Last error message on this thread ("" when none). */
 const char *md_last_error(void);
 /* Library version string. */
@@ -181,6 +200,23 @@ md_err md_batch_run(md_batch *batch, int64_t steps);
 /* md_profile_steps for the whole batch: per-phase milliseconds of one step
  * (launched directly, no graph), averaged over `steps` steps */
 md_err md_batch_profile_steps(md_batch *batch, int32_t steps, double *ms);
+
+/* The pair kernel's autotuner.  No counterpart upstream: the WebGPU loop
+ * has fixed workgroup shapes (src/gpu/loop.ts:171).  It times the pair
+ * kernel's schedules -- list entries per work unit (8, 16, 32) and resident
+ * waves (2-4 per SIMD) -- on the group's own replicas, `steps` steps each,
+ * keeps the fastest, and caches it per (replica count, atom-count bucket,
+ * triclinic or not) for launch groups created later in the process.  The
+ * schedule changes no bits; the timed steps are ordinary steps. */
+typedef struct {
+  int32_t segment_entries, pair_waves;                 /* the schedule picked */
+  double ms_per_step;                                  /* its mean step, phase by phase */
+  int32_t default_segment_entries, default_pair_waves; /* the schedule before tuning */
+  double default_ms_per_step;
+  int32_t candidates;                                  /* schedules timed */
+} md_schedule;
+md_err md_autotune(md_system *sys, int32_t steps, md_schedule *best);
+md_err md_batch_autotune(md_batch *batch, int32_t steps, md_schedule *best);
 
 /* Learned SHAKE warm start.  No counterpart upstream: it attaches where
  * `shake_cluster` starts (src/gpu/kernels/shake.wgsl:130-146), before the

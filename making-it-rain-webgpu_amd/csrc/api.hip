@@ -12,6 +12,7 @@
 #include <cstring>
 #include <map>
 #include <numeric>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -19,6 +20,9 @@
 #include "warmstart.hpp"
 
 static thread_local std::string g_last_error;
+
+// the message md_last_error reports (for entry points defined in other files)
+void md_set_last_error(const std::string &message) { g_last_error = message; }
 
 static md_err fail(const md_error &e) {
   g_last_error = e.message;
@@ -134,6 +138,33 @@ static void group_release(md_group *g) {
   delete g;
 }
 
+// ---- the pair-kernel autotuner's cache ----------------------------------------
+//
+// Tuned schedules (entries per work unit, resident waves) keyed by what sets
+// the best one: the number of replicas, the launch's atom count in
+// half-octave buckets, and whether any cell is triclinic.  A new launch group
+// whose key has an entry starts from it.
+struct md_tune_entry {
+  int seg_len, waves;
+};
+static std::map<std::tuple<int, int, int>, md_tune_entry> g_tuned;
+
+static std::tuple<int, int, int> tune_key(const md_group *g) {
+  long atoms = 0;
+  int triclinic = 0;
+  for (const md_system *s : g->reps) {
+    atoms += s->n;
+    triclinic |= (s->box[3] != 0.0 || s->box[6] != 0.0 || s->box[7] != 0.0) ? 1 : 0;
+  }
+  const int bucket = (int)std::floor(2.0 * std::log2((double)std::max(1L, atoms)));
+  return std::make_tuple((int)g->reps.size(), bucket, triclinic);
+}
+
+static void apply_tuned_schedule(md_group *g) {
+  auto it = g_tuned.find(tune_key(g));
+  if (it != g_tuned.end()) md_nb_set_schedule(g, it->second.seg_len, it->second.waves);
+}
+
 // a launch group over `reps` (all on one device), on `stream` or its own
 static md_group *group_create(const std::vector<md_system *> &reps, hipStream_t stream) {
   md_group *g = new md_group();
@@ -158,6 +189,7 @@ static md_group *group_create(const std::vector<md_system *> &reps, hipStream_t 
     g->status = md_group_alloc<uint32_t>(g, 1);
     if (any_nb) {
       md_nb_group_setup(g);
+      apply_tuned_schedule(g);
       md_pme_group_setup(g);
     }
     md_bonded_group_setup(g);
@@ -840,6 +872,58 @@ static void profile_group(md_group *g, int steps, double *ms) {
   for (md_system *s : g->reps) check_control(s);
 }
 
+// The autotuner: for each pair-kernel schedule in a small set (8, 16 or 32
+// list entries per work unit; 2, 3 or 4 resident waves per SIMD), rebuild
+// the lists with it and time `steps` steps launched phase by phase; keep the
+// fastest, record it in the cache under the group's key and report it with
+// the time of the schedule the group started with.  The schedule never
+// changes the bits (i-forces are rounded per 8 entries whatever the unit,
+// every sum is order independent), only the time; the timed steps are
+// ordinary steps of every replica.
+static void autotune_group(md_group *g, int steps, md_schedule *out) {
+  if (!g->nb) throw md_error(MD_ERR_INVALID_ARGUMENT, "a system without a NonbondedForce has no pair kernel to tune");
+  int seg0 = 0, waves0 = 0;
+  md_nb_get_schedule(g, &seg0, &waves0);
+  int cus = 1;
+  MD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
+  std::vector<std::pair<int, int>> candidates = {{seg0, waves0}};
+  for (int seg : {8, 16, 32})
+    for (int per_simd : {2, 3, 4}) {
+      std::pair<int, int> c{seg, cus * 4 * per_simd};
+      if (std::find(candidates.begin(), candidates.end(), c) == candidates.end()) candidates.push_back(c);
+    }
+  double ms[MD_PHASE_COUNT];
+  double best = 0.0, first = 0.0;
+  std::pair<int, int> pick = candidates[0];
+  for (size_t k = 0; k < candidates.size(); ++k) {
+    md_nb_set_schedule(g, candidates[k].first, candidates[k].second);
+    g->graph_stale = true;  // the pair grid is baked into a captured graph
+    for (md_system *s : g->reps) s->need_rebuild = true;
+    profile_group(g, 2, ms);  // the rebuild with the new unit length, and one more step
+    profile_group(g, steps, ms);
+    double total = 0.0;
+    for (double v : ms) total += v;
+    if (k == 0) first = total;
+    if (k == 0 || total < best) {
+      best = total;
+      pick = candidates[k];
+    }
+  }
+  md_nb_set_schedule(g, pick.first, pick.second);
+  g->graph_stale = true;
+  for (md_system *s : g->reps) s->need_rebuild = true;
+  g_tuned[tune_key(g)] = md_tune_entry{pick.first, pick.second};
+  if (out) {
+    out->segment_entries = pick.first;
+    out->pair_waves = pick.second;
+    out->ms_per_step = best;
+    out->default_segment_entries = seg0;
+    out->default_pair_waves = waves0;
+    out->default_ms_per_step = first;
+    out->candidates = (int32_t)candidates.size();
+  }
+}
+
 struct md_batch {
   md_group *group = nullptr;
 };
@@ -1134,6 +1218,21 @@ md_err md_batch_run(md_batch *b, int64_t steps) {
   if (!b || steps < 0) throw md_error(MD_ERR_INVALID_ARGUMENT, "bad arguments to md_batch_run");
   if (!b->group) throw md_error(MD_ERR_INVALID_ARGUMENT, "a system of this batch has been destroyed");
   run_group(b->group, steps);
+  MD_API_END
+}
+
+md_err md_batch_autotune(md_batch *b, int32_t steps, md_schedule *best) {
+  MD_API_BEGIN
+  if (!b || steps <= 0) throw md_error(MD_ERR_INVALID_ARGUMENT, "bad arguments to md_batch_autotune");
+  if (!b->group) throw md_error(MD_ERR_INVALID_ARGUMENT, "a system of this batch has been destroyed");
+  autotune_group(b->group, steps, best);
+  MD_API_END
+}
+
+md_err md_autotune(md_system *s, int32_t steps, md_schedule *best) {
+  MD_API_BEGIN
+  if (!s || steps <= 0) throw md_error(MD_ERR_INVALID_ARGUMENT, "bad arguments to md_autotune");
+  autotune_group(s->self, steps, best);
   MD_API_END
 }
 

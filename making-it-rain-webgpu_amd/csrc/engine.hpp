@@ -23,6 +23,9 @@ struct md_error : std::exception {
   const char *what() const noexcept override { return message.c_str(); }
 };
 
+// what md_last_error reports on this thread (set by every entry point)
+void md_set_last_error(const std::string &message);
+
 #define MD_HIP(call)                                                                \
   do {                                                                              \
     hipError_t e_ = (call);                                                         \
@@ -276,6 +279,8 @@ void md_int_group_refresh(md_group *g);
 void md_int_group_free(md_group *g);
 
 // ---- group launchers -----------------------------------------------------------
+void md_nb_set_schedule(md_group *g, int seg_len, int waves);  // pair-kernel schedule (autotuner)
+void md_nb_get_schedule(md_group *g, int *seg_len, int *waves);
 void md_nb_build(md_group *g);                  // rebuild kernels, gated per replica
 void md_nb_pairs(md_group *g, bool energy);
 void md_bonded(md_group *g, bool energy);       // bonded.hip

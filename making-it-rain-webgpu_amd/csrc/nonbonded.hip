@@ -1551,6 +1551,24 @@ void md_nb_group_setup(md_group *g) {
   md_nb_group_refresh(g);
 }
 
+// The pair kernel's schedule: list entries per work unit (a multiple of
+// MD_SEGMENT; the i-force rounding happens per MD_SEGMENT entries, so any
+// length gives the same bits) and the resident grid of waves that draws units
+// from the counter.  waves <= 0 keeps the current grid.
+void md_nb_set_schedule(md_group *g, int seg_len, int waves) {
+  auto &G = nbg(g);
+  if (seg_len % MD_SEGMENT || seg_len < MD_SEGMENT || seg_len > 8 * MD_SEGMENT)
+    throw md_error(MD_ERR_INVALID_ARGUMENT, "segment length must be 8, 16, 32 or 64 entries");
+  G.seg_len = seg_len;
+  if (waves > 0) G.pair_grid[0] = waves;
+  md_nb_group_refresh(g);
+}
+
+void md_nb_get_schedule(md_group *g, int *seg_len, int *waves) {
+  *seg_len = nbg(g).seg_len;
+  *waves = nbg(g).pair_grid[0];
+}
+
 void md_nb_group_refresh(md_group *g) {
   std::vector<md_nb_args> host;
   for (md_system *s : g->reps) host.push_back(nb_args(s, nbg(g).seg_len));

@@ -318,6 +318,11 @@ class HipBatch:
     def profileStep(self, repeats: int = 10) -> dict:
         return self._native.profile_steps(repeats)
 
+    def autotune(self, steps: int = 10) -> dict:
+        """Pick the pair kernel's schedule for this batch by timing (no effect
+        on the trajectories' bits); see ``md_batch_autotune``."""
+        return self._native.autotune(steps)
+
     @property
     def particleCount(self) -> int:
         return sum(e.particleCount for e in self.engines)

@@ -106,6 +106,15 @@ class RunOptionsC(C.Structure):
                 ("max_batch", C.c_int32)]
 
 
+class Schedule(C.Structure):
+    _fields_ = [("segment_entries", C.c_int32), ("pair_waves", C.c_int32), ("ms_per_step", C.c_double),
+                ("default_segment_entries", C.c_int32), ("default_pair_waves", C.c_int32),
+                ("default_ms_per_step", C.c_double), ("candidates", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class ListFlags(C.Structure):
     _fields_ = [("rebuilds", C.c_int64), ("tiles", C.c_int64), ("tile_capacity", C.c_int64),
                 ("worst_displacement", C.c_double), ("overflowed", C.c_int32)]
@@ -152,6 +161,8 @@ def library():
         L.md_batch_destroy.restype = None
         L.md_batch_run.argtypes = [C.c_void_p, C.c_int64]
         L.md_batch_profile_steps.argtypes = [C.c_void_p, C.c_int32, _dp]
+        L.md_autotune.argtypes = [C.c_void_p, C.c_int32, C.POINTER(Schedule)]
+        L.md_batch_autotune.argtypes = [C.c_void_p, C.c_int32, C.POINTER(Schedule)]
         _fp = P(C.c_float)
         L.md_warmstart_create.argtypes = [C.c_int32, C.c_uint32, C.c_int64, P(C.c_void_p)]
         L.md_warmstart_destroy.argtypes = [C.c_void_p]
@@ -364,6 +375,11 @@ class NativeSystem:
         check(library().md_profile_steps(self.handle, int(steps), _d(ms)))
         return {library().md_phase_name(k).decode(): float(ms[k]) for k in range(PHASE_COUNT)}
 
+    def autotune(self, steps: int) -> dict:
+        out = Schedule()
+        check(library().md_autotune(self.handle, int(steps), C.byref(out)))
+        return out.as_dict()
+
     def constrain_positions(self, tolerance: float):
         check(library().md_constrain_positions(self.handle, float(tolerance)))
 
@@ -401,6 +417,13 @@ class NativeBatch:
         ms = np.zeros(PHASE_COUNT)
         check(library().md_batch_profile_steps(self._h, int(steps), _d(ms)))
         return {library().md_phase_name(k).decode(): float(ms[k]) for k in range(PHASE_COUNT)}
+
+    def autotune(self, steps: int) -> dict:
+        """md_batch_autotune: the fastest pair-kernel schedule, and the time of
+        the one the batch started with."""
+        out = Schedule()
+        check(library().md_batch_autotune(self._h, int(steps), C.byref(out)))
+        return out.as_dict()
 
 
 PHASE_COUNT = 12

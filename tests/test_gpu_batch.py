@@ -106,3 +106,29 @@ def test_batch_profile_and_lifetime(replicas):
     batch.destroy()
     for e in engines:
         e.destroy()
+
+
+def test_autotuner_picks_a_schedule_and_the_batch_goes_on(replicas):
+    """md_batch_autotune times the pair kernel's schedules on the batch and
+    keeps the fastest (the starting schedule is among those timed, so the
+    pick is never slower); afterwards the batch runs on and its forces still
+    match the oracle."""
+    reps, states = replicas
+    engines = _engines(reps, states)
+    batch = HipBatch(engines)
+    best = batch.autotune(5)
+    print("autotuned schedule", best)
+    assert best["candidates"] >= 9
+    assert best["segment_entries"] in (8, 16, 32)
+    assert best["pair_waves"] > 0
+    assert best["ms_per_step"] <= best["default_ms_per_step"]
+    batch.run(60)
+    batch.destroy()
+    for e, r in zip(engines, reps):
+        o = O.OracleSystem(r.built.system)
+        x = e.positions()
+        _, _, f = e._native.evaluate(True)
+        _, _, of = o.potential(x, r.built.system.boxVectors, grid=e.mesh)
+        err, _ = edge.rel_rms_edge(f, of, r.built.system, x, r.built.system.boxVectors, o.alpha())
+        assert err < 2e-6
+        e.destroy()
