@@ -94,7 +94,8 @@ def oracle_rate(system, positions, velocities, budget_s, temperature=300.0, seed
     one host core: LFMiddle steps with SETTLE/SHAKE and the thermostat."""
     import oracle as O
     o = O.OracleSystem(system)
-    grid = o.default_grid(system.boxVectors)
+    stated = tuple(int(g) for g in system.force("NonbondedForce").grid)
+    grid = stated if all(stated) else o.default_grid(system.boxVectors)
     pos, vel = positions.copy(), velocities.copy()
     steps = 0
     t0 = time.perf_counter()

@@ -252,6 +252,23 @@ md_err md_warmstart_evaluate(md_warmstart *ws, const float *x, const float *t, i
 /* SHAKE update passes and cluster-steps so far, all modes */
 md_err md_shake_counts(md_system *sys, int64_t *passes, int64_t *clusters);
 
+/* StepLoop's minimiseOnDevice (src/gpu/minimise.ts:33-54) with the whole
+ * search on the device: OpenMM's LocalEnergyMinimizer objective (constraint
+ * springs stiffened tenfold per round, a give-up that restores the start,
+ * then projection to the run's constraint tolerance;
+ * src/reference/minimise.ts:164-280) searched by L-BFGS with libLBFGS's
+ * backtracking strong-Wolfe line search (src/reference/lbfgs.ts:102-275),
+ * memory 6, a 0.5 nm cap on any coordinate's move per trial.  Stops at an
+ * RMS gradient per particle <= tolerance (kJ/mol/nm) or after
+ * max_iterations (0: no limit).  Leaves the minimised positions in `sys`. */
+typedef struct {
+  double energy;               /* kJ/mol, at the final positions */
+  int32_t iterations, rounds;  /* L-BFGS iterations over all stiffening rounds */
+  int32_t gave_up;             /* 1: stiffening stopped helping, the start was restored */
+  double max_constraint_error; /* largest |r - d| / d at the end */
+} md_minimise_result;
+md_err md_minimise(md_system *sys, int32_t max_iterations, double tolerance, md_minimise_result *result);
+
 /* constrainPositions(system, x, x, tol) and constrainVelocities(system, x, v,
  * tol) (src/reference/constraints.ts:254-291) applied on the device to the
  * current positions or velocities: SETTLE for rigid waters, Jacobi SHAKE or

@@ -399,6 +399,9 @@ static void create(md_system *s, const md_system_desc *d, const md_run_options *
   }
   s->cmm_frequency = d->cmm_frequency;
   s->n_constraints = d->n_constraints;
+  s->con_p1.assign(d->constraint_p1, d->constraint_p1 + d->n_constraints);
+  s->con_p2.assign(d->constraint_p2, d->constraint_p2 + d->n_constraints);
+  s->con_d.assign(d->constraint_distance, d->constraint_distance + d->n_constraints);
   s->dof = 3 * moving - d->n_constraints - (d->cmm_frequency > 0 ? 3 : 0);
 
   // nonbonded parameters and host constants
@@ -601,6 +604,7 @@ static void release(md_system *s) {
   while (!s->groups.empty()) group_release(s->groups.back());
   if (s->nb_ext) md_nb_free(s);
   if (s->pme_ext) md_pme_free(s);
+  delete s->min_work;  // its device memory is among the system's allocations
   for (void *p : s->allocations) hipFree(p);
   if (s->stream) hipStreamDestroy(s->stream);
   delete s;
@@ -812,6 +816,196 @@ static void evaluate(md_system *s, double *forces, double *groups, double *total
       for (int d = 0; d < 3; ++d) forces[3 * i + d] = md_fixed_to_double(f[(size_t)d * s->np + i]);
 }
 
+// ---- the device-resident minimiser --------------------------------------------
+//
+// minimiseOnDevice (src/gpu/minimise.ts:33-54) with its search on the device:
+// OpenMM's LocalEnergyMinimizer objective U(x) + sum (k/2)(r - d)^2 over the
+// constraints, k = 100 / max(1e-4, tol), stiffened tenfold per round until
+// every constraint is within max(1e-4, tol), giving up to the starting
+// positions when stiffening stops helping, then projected to the run's own
+// tolerance (src/reference/minimise.ts:164-280).  The search is L-BFGS with
+// libLBFGS's backtracking strong-Wolfe line search as the reference restates
+// it (src/reference/lbfgs.ts:102-275): memory 6, first step 1/|d|, then 1,
+// x0.5 on failed decrease or overshoot, x2.1 while still steep, Armijo 1e-4,
+// curvature 0.9, 40 trials, stop at RMS gradient per particle <= tolerance;
+// plus a 0.5 nm cap on any coordinate's move per trial.  All 3N-vectors stay
+// on the device; the host reads back energies, dot products and maxima.
+
+namespace {
+struct lbfgs_state {
+  md_system *s;
+  md_min_work *w;
+  double kspring;
+  double *x, *g, *d, *trial, *gt, *q;
+  double *S[MD_MIN_MEMORY], *Y[MD_MIN_MEMORY];
+};
+}  // namespace
+
+static double min_objective(lbfgs_state &L, const double *x, double *g) {
+  md_system *s = L.s;
+  if (!md_min_positions_in(s, L.w, x))
+    throw md_error(MD_ERR_FIXED_POINT_OVERFLOW, "the minimiser moved a coordinate outside the fixed-point range");
+  MD_HIP(hipMemsetAsync(&s->ctl->overflow, 0, sizeof(uint32_t), s->stream));
+  s->need_rebuild = true;
+  double total = 0.0;
+  evaluate(s, nullptr, nullptr, &total, false);
+  return total + md_min_gradient(s, L.w, x, L.kspring, g);
+}
+
+// returns the iterations taken; L.x holds the last accepted point
+static int min_lbfgs(lbfgs_state &L, int max_iterations, double rms_tolerance, double &value) {
+  md_system *s = L.s;
+  const double max_move = 0.5, armijo = 1e-4, curvature = 0.9;
+  const int max_line_search = 40;
+  auto done = [&](const double *grad) { return std::sqrt(md_min_dot(s, L.w, grad, grad) / s->n) <= rms_tolerance; };
+  value = min_objective(L, L.x, L.g);
+  if (done(L.g)) return 0;
+  double rho[MD_MIN_MEMORY];
+  int count = 0, head = 0;  // pairs held; the slot the next pair goes to
+  md_min_lincomb(s, L.d, -1.0, L.g, 0.0, nullptr);
+  double step = 1.0 / std::sqrt(md_min_dot(s, L.w, L.d, L.d));
+  int iteration = 0;
+  for (;;) {
+    ++iteration;
+    const double slope0 = md_min_dot(s, L.w, L.g, L.d);
+    if (slope0 >= 0.0) return iteration - 1;
+    bool accepted = false;
+    double v = 0.0;
+    const double dmax = md_min_absmax(s, L.w, L.d);
+    for (int trial = 0; trial < max_line_search; ++trial) {
+      if (step * dmax > max_move) {
+        step *= 0.5;
+        continue;
+      }
+      md_min_lincomb(s, L.trial, 1.0, L.x, step, L.d);
+      v = min_objective(L, L.trial, L.gt);
+      if (!(v <= value + armijo * step * slope0)) {
+        step *= 0.5;
+        continue;
+      }
+      const double slope = md_min_dot(s, L.w, L.gt, L.d);
+      if (slope < curvature * slope0) {
+        if (2.1 * step * dmax > max_move) {
+          accepted = true;  // still steep, but at the displacement cap
+          break;
+        }
+        step *= 2.1;
+      } else if (slope > -curvature * slope0) {
+        step *= 0.5;
+      } else {
+        accepted = true;
+        break;
+      }
+    }
+    if (!accepted) return iteration - 1;
+    // the correction pair of this step, into the next slot of the ring
+    md_min_lincomb(s, L.S[head], 1.0, L.trial, -1.0, L.x);
+    md_min_lincomb(s, L.Y[head], 1.0, L.gt, -1.0, L.g);
+    std::swap(L.x, L.trial);
+    std::swap(L.g, L.gt);
+    value = v;
+    if (done(L.g) || (max_iterations > 0 && iteration >= max_iterations)) return iteration;
+    const double sy = md_min_dot(s, L.w, L.S[head], L.Y[head]);
+    if (sy > 0.0) {
+      rho[head] = 1.0 / sy;
+      head = (head + 1) % MD_MIN_MEMORY;
+      count = std::min(count + 1, MD_MIN_MEMORY);
+    }
+    // two-loop recursion: q = -H g
+    md_min_lincomb(s, L.q, -1.0, L.g, 0.0, nullptr);
+    double alpha[MD_MIN_MEMORY];
+    for (int k = 0; k < count; ++k) {  // newest to oldest
+      const int slot = (head - 1 - k + 2 * MD_MIN_MEMORY) % MD_MIN_MEMORY;
+      alpha[k] = rho[slot] * md_min_dot(s, L.w, L.S[slot], L.q);
+      md_min_lincomb(s, L.q, 1.0, L.q, -alpha[k], L.Y[slot]);
+    }
+    if (count) {
+      const int newest = (head - 1 + MD_MIN_MEMORY) % MD_MIN_MEMORY;
+      md_min_lincomb(s, L.q, 1.0 / (rho[newest] * md_min_dot(s, L.w, L.Y[newest], L.Y[newest])), L.q, 0.0,
+                     nullptr);
+    }
+    for (int k = count - 1; k >= 0; --k) {  // oldest to newest
+      const int slot = (head - 1 - k + 2 * MD_MIN_MEMORY) % MD_MIN_MEMORY;
+      const double beta = rho[slot] * md_min_dot(s, L.w, L.Y[slot], L.q);
+      md_min_lincomb(s, L.q, 1.0, L.q, alpha[k] - beta, L.S[slot]);
+    }
+    std::swap(L.d, L.q);
+    step = 1.0;
+  }
+}
+
+static void minimise_system(md_system *s, int max_iterations, double tolerance, md_minimise_result *out) {
+  MD_HIP(hipSetDevice(s->device));
+  if (!s->min_work) s->min_work = md_min_work_create(s, s->con_p1, s->con_p2, s->con_d);
+  md_min_work *w = s->min_work;
+  lbfgs_state L;
+  L.s = s;
+  L.w = w;
+  L.x = md_min_vec(s, w, 0);
+  L.g = md_min_vec(s, w, 1);
+  L.d = md_min_vec(s, w, 2);
+  L.trial = md_min_vec(s, w, 3);
+  L.gt = md_min_vec(s, w, 4);
+  L.q = md_min_vec(s, w, 5);
+  for (int k = 0; k < MD_MIN_MEMORY; ++k) {
+    L.S[k] = md_min_vec(s, w, 6 + k);
+    L.Y[k] = md_min_vec(s, w, 6 + MD_MIN_MEMORY + k);
+  }
+  const double ctol = s->opt.constraint_tolerance, work = std::max(1e-4, ctol);
+  L.kspring = 100.0 / work;
+  md_constrain(s->self, 0, work);
+  MD_HIP(hipStreamSynchronize(s->stream));
+  // the starting point, kept in the last correction slot until the search
+  // needs it (a give-up restores it): copy it aside in host memory instead,
+  // so every device vector stays free for the search
+  std::vector<double> initial(3 * (size_t)s->np);
+  md_min_positions_out(s, L.x);
+  MD_HIP(hipMemcpyAsync(initial.data(), L.x, sizeof(double) * initial.size(), hipMemcpyDeviceToHost, s->stream));
+  MD_HIP(hipStreamSynchronize(s->stream));
+  auto restore = [&]() {
+    MD_HIP(hipMemcpyAsync(L.x, initial.data(), sizeof(double) * initial.size(), hipMemcpyHostToDevice, s->stream));
+    MD_HIP(hipStreamSynchronize(s->stream));
+  };
+  double prev1 = 1e10, prev2 = 1e10, value = 0.0;
+  int rounds = 0, iterations = 0;
+  bool gave_up = false;
+  for (;;) {
+    ++rounds;
+    iterations += min_lbfgs(L, max_iterations, tolerance, value);
+    const double err = md_min_constraint_error(s, w, L.x);
+    if (err <= work || w->nc == 0) break;
+    if (err >= prev2) {
+      restore();
+      gave_up = true;
+      break;
+    }
+    prev2 = prev1;
+    prev1 = err;
+    L.kspring *= 10.0;
+    if (err > 100 * work) restore();
+  }
+  if (!md_min_positions_in(s, w, L.x))
+    throw md_error(MD_ERR_FIXED_POINT_OVERFLOW, "the minimiser moved a coordinate outside the fixed-point range");
+  MD_HIP(hipMemsetAsync(&s->ctl->overflow, 0, sizeof(uint32_t), s->stream));
+  MD_HIP(hipMemsetAsync(&s->ctl->diverged, 0, sizeof(uint32_t), s->stream));
+  s->need_rebuild = true;
+  if (ctol < work) {
+    md_constrain(s->self, 0, ctol);
+    MD_HIP(hipStreamSynchronize(s->stream));
+  }
+  double total = 0.0;
+  evaluate(s, nullptr, nullptr, &total, false);
+  md_min_positions_out(s, L.x);
+  const double err = md_min_constraint_error(s, w, L.x);
+  if (out) {
+    out->energy = total;
+    out->iterations = iterations;
+    out->rounds = rounds;
+    out->gave_up = gave_up ? 1 : 0;
+    out->max_constraint_error = err;
+  }
+}
+
 // ---- probes ------------------------------------------------------------------
 
 __global__ void k_probe_fixed(const float *in, int n, int64_t *fixed, float *back) {
@@ -930,6 +1124,18 @@ struct md_batch {
 
 // ---- the C ABI -------------------------------------------------------------
 
+// A probe's device staging, freed when the call returns or throws.
+template <typename T>
+struct probe_buffer {
+  T *p = nullptr;
+  explicit probe_buffer(size_t n) { MD_HIP(hipMalloc(&p, sizeof(T) * n)); }
+  ~probe_buffer() {
+    if (p) (void)hipFree(p);
+  }
+  probe_buffer(const probe_buffer &) = delete;
+  probe_buffer &operator=(const probe_buffer &) = delete;
+};
+
 extern "C" {
 
 const char *md_last_error(void) { return g_last_error.c_str(); }
@@ -1047,6 +1253,13 @@ md_err md_run(md_system *s, int64_t steps) {
   MD_API_END
 }
 
+md_err md_minimise(md_system *s, int32_t max_iterations, double tolerance, md_minimise_result *result) {
+  MD_API_BEGIN
+  if (!s || max_iterations < 0 || !(tolerance > 0)) throw md_error(MD_ERR_INVALID_ARGUMENT, "bad arguments to md_minimise");
+  minimise_system(s, max_iterations, tolerance, result);
+  MD_API_END
+}
+
 md_err md_constrain_positions(md_system *s, double tolerance) {
   MD_API_BEGIN
   if (!s || !(tolerance > 0)) throw md_error(MD_ERR_INVALID_ARGUMENT, "bad arguments to md_constrain_positions");
@@ -1130,20 +1343,14 @@ md_err md_probe_fixed_point(const float *values, int32_t n, int64_t *fixed, floa
   if (n == 0) return MD_OK;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw md_error(MD_ERR_NO_DEVICE, "no HIP device");
-  float *din;
-  int64_t *dfix;
-  float *dback;
-  MD_HIP(hipMalloc(&din, sizeof(float) * n));
-  MD_HIP(hipMalloc(&dfix, sizeof(int64_t) * n));
-  MD_HIP(hipMalloc(&dback, sizeof(float) * n));
-  MD_HIP(hipMemcpy(din, values, sizeof(float) * n, hipMemcpyHostToDevice));
-  k_probe_fixed<<<(n + 255) / 256, 256>>>(din, n, dfix, dback);
+  probe_buffer<float> din(n);
+  probe_buffer<int64_t> dfix(n);
+  probe_buffer<float> dback(n);
+  MD_HIP(hipMemcpy(din.p, values, sizeof(float) * n, hipMemcpyHostToDevice));
+  k_probe_fixed<<<(n + 255) / 256, 256>>>(din.p, n, dfix.p, dback.p);
   MD_HIP(hipGetLastError());
-  MD_HIP(hipMemcpy(fixed, dfix, sizeof(int64_t) * n, hipMemcpyDeviceToHost));
-  MD_HIP(hipMemcpy(back, dback, sizeof(float) * n, hipMemcpyDeviceToHost));
-  hipFree(din);
-  hipFree(dfix);
-  hipFree(dback);
+  MD_HIP(hipMemcpy(fixed, dfix.p, sizeof(int64_t) * n, hipMemcpyDeviceToHost));
+  MD_HIP(hipMemcpy(back, dback.p, sizeof(float) * n, hipMemcpyDeviceToHost));
   MD_API_END
 }
 
@@ -1153,16 +1360,12 @@ md_err md_probe_thermostat(uint32_t seed, uint32_t step, int32_t n, uint32_t *bi
   if (n == 0) return MD_OK;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw md_error(MD_ERR_NO_DEVICE, "no HIP device");
-  uint32_t *db;
-  float *dn;
-  MD_HIP(hipMalloc(&db, sizeof(uint32_t) * 4 * n));
-  MD_HIP(hipMalloc(&dn, sizeof(float) * 3 * n));
-  k_probe_thermostat<<<(n + 255) / 256, 256>>>(seed, step, n, db, dn);
+  probe_buffer<uint32_t> db(4 * (size_t)n);
+  probe_buffer<float> dn(3 * (size_t)n);
+  k_probe_thermostat<<<(n + 255) / 256, 256>>>(seed, step, n, db.p, dn.p);
   MD_HIP(hipGetLastError());
-  MD_HIP(hipMemcpy(bits, db, sizeof(uint32_t) * 4 * n, hipMemcpyDeviceToHost));
-  MD_HIP(hipMemcpy(normals, dn, sizeof(float) * 3 * n, hipMemcpyDeviceToHost));
-  hipFree(db);
-  hipFree(dn);
+  MD_HIP(hipMemcpy(bits, db.p, sizeof(uint32_t) * 4 * n, hipMemcpyDeviceToHost));
+  MD_HIP(hipMemcpy(normals, dn.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost));
   MD_API_END
 }
 

@@ -93,6 +93,7 @@ __device__ __forceinline__ int md_locate(const md_bmap &m, int blk, int &local) 
 struct md_system;
 struct md_ws_net;
 struct md_warmstart;
+struct md_min_work;
 
 // Host side of a block map: per-replica block counts -> device prefix.
 struct md_launch {
@@ -229,6 +230,11 @@ struct md_system {
   float *rec_x = nullptr, *rec_t = nullptr;  // sample ring of a recording run
   int64_t rec_first_step = 0, rec_cap = 0;
 
+  // ---- the minimiser's work vectors, made on the first md_minimise ----------
+  md_min_work *min_work = nullptr;
+  std::vector<int> con_p1, con_p2;    // the constraints (host copies)
+  std::vector<double> con_d;
+
   // ---- run state ---------------------------------------------------------
   md_group *self = nullptr;          // this system as a group of one
   std::vector<md_group *> groups;    // every group it is in (its own first)
@@ -292,6 +298,30 @@ void md_step_prologue(md_group *g, bool advance);
 void md_integrate(md_group *g);
 void md_momentum(md_group *g);
 void md_constrain(md_group *g, int what, double tolerance);  // 0 positions, 1 velocities
+
+// ---- the device-resident minimiser (minimise.hip) -----------------------------
+// L-BFGS work vectors: MD_MIN_VECTORS f64 3N-vectors in planes [3][np]
+#define MD_MIN_MEMORY 6
+#define MD_MIN_VECTORS (6 + 2 * MD_MIN_MEMORY)
+struct md_min_work {
+  double *vec = nullptr;      // [MD_MIN_VECTORS][3][np]
+  double *partial = nullptr;  // reduction partials
+  double *scalar = nullptr;   // one reduced scalar
+  int *bad = nullptr;         // a coordinate outside the fixed-point range
+  int nc = 0;                 // live constraints (springs)
+  int *p1 = nullptr, *p2 = nullptr;
+  double *dist = nullptr;
+};
+md_min_work *md_min_work_create(md_system *s, const std::vector<int> &p1, const std::vector<int> &p2,
+                                const std::vector<double> &dist);
+double *md_min_vec(md_system *s, md_min_work *w, int k);
+double md_min_dot(md_system *s, md_min_work *w, const double *a, const double *b);
+double md_min_absmax(md_system *s, md_min_work *w, const double *a);
+void md_min_lincomb(md_system *s, double *out, double a, const double *x, double b, const double *y);
+void md_min_positions_out(md_system *s, double *x);
+bool md_min_positions_in(md_system *s, md_min_work *w, const double *x);
+double md_min_gradient(md_system *s, md_min_work *w, const double *x, double kspring, double *g);
+double md_min_constraint_error(md_system *s, md_min_work *w, const double *x);
 
 // ---- learned SHAKE warm start (warmstart.hip) ---------------------------------
 md_warmstart *md_ws_create(int device, uint32_t seed, int64_t capacity);

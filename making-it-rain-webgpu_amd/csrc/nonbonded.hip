@@ -922,8 +922,8 @@ __device__ __forceinline__ int4 md_uniform_xyz(const int4 *p) {
 }
 
 // DPP lane moves (row = 16 lanes): quad_perm [1,0,3,2] and [2,3,0,1] give
-// lane l the value of l^1 and l^2; row_ror:n the value of lane (l+n) mod 16
-// of its row
+// lane l the value of l^1 and l^2; row_ror:n (rotate right by n) the value of
+// lane (l-n) mod 16 of its row
 template <int CTRL>
 __device__ __forceinline__ float md_dpp(float x) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, false));
@@ -949,19 +949,20 @@ __device__ __forceinline__ float md_add_xor32(float x) {
 // on each: ((x + x^8) + (.)^16) + (.)^32, so the bits match a shuffle
 // butterfly of the same order.
 __device__ __forceinline__ float md_sum_same_j(float x) {
-  x += md_dpp<MD_DPP_ROR8>(x);  // within a row, lane (l+8) mod 16 is l^8
+  x += md_dpp<MD_DPP_ROR8>(x);  // within a row, lane (l-8) mod 16 is l^8
   x = md_add_xor16(x);
   return md_add_xor32(x);
 }
 
 // The sum over the 8 lanes sharing l/8, valid on the lanes with l%8 in
 // 0-3 (HIGH false) or 4-7 (HIGH true): the butterfly's ((x + x^1) + (.)^2)
-// then + (.)^4, whose partner l^4 is lane l+4 (l%8 < 4) or l-4 (l%8 >= 4).
+// then + (.)^4, whose partner l^4 is lane l+4 (l%8 < 4: rotate right by 12)
+// or l-4 (l%8 >= 4: rotate right by 4).
 template <bool HIGH>
 __device__ __forceinline__ float md_sum_same_i(float x) {
   x += md_dpp<MD_DPP_XOR1>(x);
   x += md_dpp<MD_DPP_XOR2>(x);
-  return x + md_dpp<HIGH ? MD_DPP_ROR12 : MD_DPP_ROR4>(x);
+  return x + md_dpp<HIGH ? MD_DPP_ROR4 : MD_DPP_ROR12>(x);
 }
 
 // gap between boxes [ilo, ihi] and [jlo, jhi] on one axis, in frame units

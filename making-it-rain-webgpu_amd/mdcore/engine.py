@@ -226,7 +226,12 @@ class HipEngine:
                 "totalEnergy": r.totalEnergy, "temperature": r.temperature}
 
     def minimise(self, maxIterations: int = 0, tolerance: float = 10.0) -> float:
-        return minimise(self, maxIterations, tolerance).energy
+        """ProtocolEngine.minimise: the device-resident minimiser (md_minimise);
+        the module's ``minimise`` runs the same search on the host."""
+        r = self._native.minimise(maxIterations, tolerance)
+        self.lastMinimisation = MinimiseResult(r["energy"], r["iterations"], r["rounds"], bool(r["gave_up"]),
+                                               r["max_constraint_error"])
+        return r["energy"]
 
     # -- StepLoop extras ------------------------------------------------------
     def potential(self, positions=None):

@@ -115,6 +115,11 @@ class Schedule(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class MinimiseResultC(C.Structure):
+    _fields_ = [("energy", C.c_double), ("iterations", C.c_int32), ("rounds", C.c_int32), ("gave_up", C.c_int32),
+                ("max_constraint_error", C.c_double)]
+
+
 class ListFlags(C.Structure):
     _fields_ = [("rebuilds", C.c_int64), ("tiles", C.c_int64), ("tile_capacity", C.c_int64),
                 ("worst_displacement", C.c_double), ("overflowed", C.c_int32)]
@@ -162,6 +167,7 @@ def library():
         L.md_batch_run.argtypes = [C.c_void_p, C.c_int64]
         L.md_batch_profile_steps.argtypes = [C.c_void_p, C.c_int32, _dp]
         L.md_autotune.argtypes = [C.c_void_p, C.c_int32, C.POINTER(Schedule)]
+        L.md_minimise.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.POINTER(MinimiseResultC)]
         L.md_batch_autotune.argtypes = [C.c_void_p, C.c_int32, C.POINTER(Schedule)]
         _fp = P(C.c_float)
         L.md_warmstart_create.argtypes = [C.c_int32, C.c_uint32, C.c_int64, P(C.c_void_p)]
@@ -379,6 +385,12 @@ class NativeSystem:
         out = Schedule()
         check(library().md_autotune(self.handle, int(steps), C.byref(out)))
         return out.as_dict()
+
+    def minimise(self, max_iterations: int = 0, tolerance: float = 10.0) -> dict:
+        """md_minimise: L-BFGS with constraint springs, searched on the device."""
+        out = MinimiseResultC()
+        check(library().md_minimise(self.handle, int(max_iterations), float(tolerance), C.byref(out)))
+        return {k: getattr(out, k) for k, _ in out._fields_}
 
     def constrain_positions(self, tolerance: float):
         check(library().md_constrain_positions(self.handle, float(tolerance)))

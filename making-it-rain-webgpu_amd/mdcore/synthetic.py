@@ -65,9 +65,11 @@ class _Topology:
         self.exc: List[Tuple[int, int, float, float, float]] = []
         self.heavy: List[int] = []
         self.grid = {}
+        self.tpos: List[Tuple[float, float, float]] = []  # pos as Python floats, for the distance scans
 
     def add(self, p, m, q, s, e):
         self.pos.append(np.asarray(p, dtype=np.float64))
+        self.tpos.append((float(self.pos[-1][0]), float(self.pos[-1][1]), float(self.pos[-1][2])))
         self.mass.append(m)
         self.q.append(q)
         self.sig.append(s)
@@ -82,7 +84,7 @@ class _Topology:
         """Drop the atoms from index n on and the constraints from n_cons on."""
         for i in range(n, len(self.pos)):
             self.grid[key(self.pos[i])].remove(i)
-        for lst in (self.pos, self.mass, self.q, self.sig, self.eps):
+        for lst in (self.pos, self.tpos, self.mass, self.q, self.sig, self.eps):
             del lst[n:]
         del self.cons[n_cons:]
 
@@ -125,12 +127,19 @@ def _chain(top: _Topology, rng, n_atoms: int, center, radius: float, wall: float
     def nearest(p, skip):
         k = key(p)
         best = 1.0
+        p0, p1, p2 = float(p[0]), float(p[1]), float(p[2])
+        tpos, grid = top.tpos, top.grid
         for dx in (-1, 0, 1):
             for dy in (-1, 0, 1):
                 for dz in (-1, 0, 1):
-                    for j in top.grid.get((k[0] + dx, k[1] + dy, k[2] + dz), ()):
+                    for j in grid.get((k[0] + dx, k[1] + dy, k[2] + dz), ()):
                         if j not in skip:
-                            best = min(best, float(np.sum((top.pos[j] - p) ** 2)))
+                            # (d0^2 + d1^2) + d2^2: the order numpy's sum of three takes
+                            a = tpos[j]
+                            d0, d1, d2 = a[0] - p0, a[1] - p1, a[2] - p2
+                            d = d0 * d0 + d1 * d1 + d2 * d2
+                            if d < best:
+                                best = d
         return math.sqrt(best)
 
     def clash(p, skip, limit=0.25):
