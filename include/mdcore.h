@@ -319,6 +319,14 @@ md_err md_pair_census(md_system *sys, int64_t *pairs_within_cutoff, int64_t *til
  * fixed-point twin): run the device primitives on host-given inputs. */
 md_err md_probe_fixed_point(const float *values, int32_t n, int64_t *fixed, float *back);
 md_err md_probe_thermostat(uint32_t seed, uint32_t step, int32_t n, uint32_t *bits, float *normals);
+/* No counterpart in the reference: md_probe_fixed_point for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and results; the input
+ * and both outputs are staged in a fixed 1 MiB device array that each GPU
+ * keeps until the process ends, so n is at most 65536 (more is
+ * MD_ERR_INVALID_ARGUMENT).  Calls no hipMalloc/hipFree.  Calls that overlap
+ * on one GPU share the array and race. */
+md_err md_probe_fixed_point_noalloc(const float *values, int32_t n, int64_t *fixed, float *back);
 /* The order in which the neighbour search sorts cells (host only, no
  * device): cells[k] = (x * dims[1] + y) * dims[2] + z of the k-th cell of a
  * dims[0] x dims[1] x dims[2] grid.  Stands in for the counting sort's cell

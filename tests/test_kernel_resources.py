@@ -21,7 +21,8 @@ CSRC = os.path.join(ROOT, "making-it-rain-webgpu_amd", "csrc")
 HIPCC = "/opt/rocm/bin/hipcc"
 FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
          "-packed-fp32-ops", "--cuda-device-only", "-S"]
-SOURCES = ["api.hip", "nonbonded.hip", "bonded.hip", "pme.hip", "integrate.hip", "warmstart.hip", "minimise.hip"]
+SOURCES = ["api.hip", "nonbonded.hip", "bonded.hip", "pme.hip", "integrate.hip", "warmstart.hip", "minimise.hip",
+           "noalloc.hip"]
 
 
 def _report(src, out_dir):
@@ -29,7 +30,10 @@ def _report(src, out_dir):
     subprocess.run([HIPCC, *FLAGS, "-o", out, os.path.join(CSRC, src)], check=True, capture_output=True)
     text = open(out).read()
     kernels = {}
+    functions = set(re.findall(r"\.type\s+(_Z\w+),@function", text))
     for m in re.finditer(r"^(_Z\w+):", text, re.M):
+        if m.group(1) not in functions:
+            continue  # a device variable's label
         tail = text[m.end():]
         sc = re.search(r"; ScratchSize: (\d+)", tail)
         kernels[m.group(1)] = int(sc.group(1))
