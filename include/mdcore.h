@@ -178,6 +178,45 @@ md_err md_evaluate(md_system *sys, double *forces, double *groups, double *total
  * lives on the device. */
 md_err md_run(md_system *sys, int64_t steps);
 
+/* The Monte Carlo barostat on the device.  Upstream attempts the move on the
+ * host between submissions (StepLoop.attemptBarostat, src/gpu/loop.ts:602-641;
+ * barostat.ts:129-274), cutting every batch at an attempt.  Attached here, the
+ * whole attempt runs on the device inside md_run's captured strides: the
+ * energy, the Threefry BAROSTAT_DOMAIN draws (counter, seed), the isotropic
+ * trial box (V' = V + s (2u - 1), lengths times (V'/V)^(1/3), reduced), each
+ * molecule's unweighted centroid scaled with its atoms moved rigidly, every
+ * box-dependent constant, the second energy, the Metropolis test on
+ * w = dE + P dV - N_mol kT ln(V'/V) and the proposal-width adaptation every 10
+ * attempts.  The attempt fires after every `frequency` steps counted across
+ * runs, as upstream; the PME mesh stays fixed.  A system with a barostat
+ * attached cannot join a batch. */
+typedef struct {
+  double pressure;           /* bar */
+  double temperature;        /* K */
+  int32_t frequency;         /* steps between attempts; 0 detaches */
+  int32_t n_molecules;
+  const int32_t *molecule_start; /* [n_molecules + 1], molecule_start[n_molecules] = n_particles */
+  const int32_t *molecule_atoms; /* every particle once, ascending within each molecule */
+  uint32_t seed;
+  /* starting state (a resumed run passes what md_barostat_get returned) */
+  double volume_scale;       /* proposal half-width, nm^3; 0: 1% of the volume */
+  int64_t draws;             /* Threefry counter */
+  int32_t steps_since_attempt;
+  int32_t window_attempted, window_accepted;
+} md_barostat_desc;
+md_err md_barostat_attach(md_system *sys, const md_barostat_desc *desc);
+
+typedef struct {
+  int64_t attempts, accepted;      /* since attached */
+  int64_t draws;
+  int32_t steps_since_attempt;
+  int32_t window_attempted, window_accepted;
+  double volume_scale;             /* nm^3 */
+  int32_t last_accepted;           /* the latest attempt */
+  double last_energy_before, last_energy_after, last_work, last_length_scale;  /* kJ/mol */
+} md_barostat_state;
+md_err md_barostat_get(md_system *sys, md_barostat_state *state);
+
 /* Steps taken so far, and StepLoop.advanceStepCount (src/gpu/loop.ts:554-569). */
 md_err md_get_step(md_system *sys, int64_t *step);
 md_err md_set_step(md_system *sys, int64_t step);

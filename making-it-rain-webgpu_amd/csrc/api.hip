@@ -50,14 +50,6 @@ static const int STRIDE_STEPS = 50;
 
 // ---- cell geometry (src/reference/triclinic.ts) ------------------------------
 
-// ties to even, as reduceBox (triclinic.ts:93-125)
-static double reduce_by(double ratio) {
-  double nearest = std::floor(ratio + 0.5);
-  if (std::fabs(ratio - std::trunc(ratio)) == 0.5 && std::fmod(nearest, 2.0) != 0.0)
-    return nearest - (ratio > 0 ? 1.0 : -1.0);
-  return nearest;
-}
-
 static void reduce_box(const double *in, double *out) {
   double v[9];
   std::memcpy(v, in, sizeof v);
@@ -66,25 +58,16 @@ static void reduce_box(const double *in, double *out) {
       throw md_error(MD_ERR_INVALID_ARGUMENT, "box vectors must be lower triangular: a along x, b in the xy plane");
   if (v[0] <= 0 || v[4] <= 0 || v[8] <= 0)
     throw md_error(MD_ERR_INVALID_ARGUMENT, "box vectors need positive diagonal elements");
-  double t = reduce_by(v[7] / v[4]);
-  for (int k = 0; k < 3; ++k) v[6 + k] -= t * v[3 + k];
-  t = reduce_by(v[6] / v[0]);
-  for (int k = 0; k < 3; ++k) v[6 + k] -= t * v[k];
-  t = reduce_by(v[3] / v[0]);
-  for (int k = 0; k < 3; ++k) v[3 + k] -= t * v[k];
-  v[1] = v[2] = v[5] = 0.0;
+  md_reduce_box(v);  // ties to even, as reduceBox (triclinic.ts:93-125)
   std::memcpy(out, v, sizeof v);
 }
 
-static double box_volume(const double *b) { return b[0] * b[4] * b[8]; }
+static double box_volume(const double *b) { return md_box_volume(b); }
 
 static double min_perpendicular_width(const double *b) {
-  double v = box_volume(b);
-  auto cn = [](const double *u, const double *w) {
-    double x = u[1] * w[2] - u[2] * w[1], y = u[2] * w[0] - u[0] * w[2], z = u[0] * w[1] - u[1] * w[0];
-    return std::sqrt(x * x + y * y + z * z);
-  };
-  return std::min(v / cn(b + 3, b + 6), std::min(v / cn(b + 6, b), v / cn(b, b + 3)));
+  double w[3];
+  md_box_widths(b, w);
+  return std::min(w[0], std::min(w[1], w[2]));
 }
 
 static int64_t fixed_checked(double x, const char *what) {
@@ -213,25 +196,12 @@ static void apply_box(md_system *s, const double *in) {
                      "a " + std::to_string(s->cutoff) + " nm cutoff does not fit this box: the minimum image is only "
                      "exact below " + std::to_string(maxc) + " nm (half the narrowest perpendicular width)");
   }
+  for (int k = 0; k < 9; ++k) (void)fixed_checked(b[k], "box");
   std::memcpy(s->box, b, sizeof b);
   md_box &d = s->dbox;
   std::memset(&d, 0, sizeof d);
-  for (int k = 0; k < 3; ++k) {
-    d.fa[k] = fixed_checked(b[k], "box");
-    d.fb[k] = fixed_checked(b[3 + k], "box");
-    d.fc[k] = fixed_checked(b[6 + k], "box");
-    d.a[k] = (float)b[k];
-    d.b[k] = (float)b[3 + k];
-    d.c[k] = (float)b[6 + k];
-  }
-  double ax = b[0], bx = b[3], by = b[4], cx = b[6], cy = b[7], cz = b[8];
-  double m[9] = {1.0 / ax, 0, 0, -bx / (ax * by), 1.0 / by, 0, ((bx * cy) / (by * cz) - cx / cz) / ax,
-                 -cy / (by * cz), 1.0 / cz};
-  for (int k = 0; k < 9; ++k) {
-    d.invd[k] = m[k];
-    d.inv[k] = (float)m[k];
-  }
-  d.rectangular = bx == 0 && cx == 0 && cy == 0;
+  bool bad = false;
+  md_box_from_matrix(b, d, bad);
   // the kernels read the box from their groups' tables: refresh them (a
   // captured stride keeps pointing at the tables, so it stays valid)
   for (md_group *g : s->groups) group_refresh(g);
@@ -605,6 +575,7 @@ static void release(md_system *s) {
   if (s->nb_ext) md_nb_free(s);
   if (s->pme_ext) md_pme_free(s);
   delete s->min_work;  // its device memory is among the system's allocations
+  md_baro_release(s);
   for (void *p : s->allocations) hipFree(p);
   if (s->stream) hipStreamDestroy(s->stream);
   delete s;
@@ -659,6 +630,12 @@ static void check_control(md_system *s) {
                    "the cluster-pair list outgrew its capacity (the system compressed far beyond its starting "
                    "density); set positions at a lower density or recreate the system from the current state" +
                        when);
+  if (c.barostat_error == 1)
+    throw md_error(MD_ERR_CUTOFF_TOO_LARGE,
+                   "a barostat move shrank the box below twice the cutoff along its narrowest width" + when);
+  if (c.barostat_error)
+    throw md_error(MD_ERR_FIXED_POINT_OVERFLOW,
+                   "a barostat move put an atom outside the fixed-point range of +/-2^31 nm" + when);
   if (c.diverged)
     throw md_error(MD_ERR_CONSTRAINTS_DIVERGED,
                    "the constraints diverged" + when +
@@ -702,10 +679,12 @@ static void drop_graphs(md_group *g) {
   }
 }
 
-// the captured graph of `steps` steps: slot 0 holds whole strides, slot 1 the
-// last shorter run (captured again when a run of another length comes)
-static hipGraphExec_t stride_graph(md_group *g, int steps, int stride) {
-  md_group::captured &c = g->graphs[steps == stride ? 0 : 1];
+// the captured graph of `steps` steps, followed by a barostat attempt when
+// `attempt` (the group is then one system carrying a barostat): slots 0 and 2
+// hold whole strides, slots 1 and 3 the last shorter run (captured again when
+// a run of another length comes)
+static hipGraphExec_t stride_graph(md_group *g, int steps, int stride, bool attempt) {
+  md_group::captured &c = g->graphs[(steps == stride ? 0 : 1) + (attempt ? 2 : 0)];
   if (c.exec && c.steps == steps) return c.exec;
   if (c.exec) MD_HIP(hipGraphExecDestroy(c.exec));
   if (c.graph) MD_HIP(hipGraphDestroy(c.graph));
@@ -713,6 +692,7 @@ static hipGraphExec_t stride_graph(md_group *g, int steps, int stride) {
   MD_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
   try {
     for (int k = 0; k < steps; ++k) launch_step(g);
+    if (attempt) md_baro_attempt(g->reps[0]);
   } catch (...) {
     hipGraph_t broken = nullptr;
     (void)hipStreamEndCapture(g->stream, &broken);
@@ -722,6 +702,7 @@ static hipGraphExec_t stride_graph(md_group *g, int steps, int stride) {
   MD_HIP(hipStreamEndCapture(g->stream, &c.graph));
   MD_HIP(hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0));
   c.steps = steps;
+  c.attempt = attempt;
   return c.exec;
 }
 
@@ -730,6 +711,12 @@ static hipGraphExec_t stride_graph(md_group *g, int steps, int stride) {
 // any replica's control block holds an error -- is read, so a run whose list
 // overflows or whose constraints diverge stops within one stride of it.  A
 // run that ends in an error leaves every replica's step count unchanged.
+//
+// A system carrying a device barostat (always a group of one) cuts its chunks
+// at the attempts: every `frequency` steps counted across runs, as upstream's
+// runBatch (src/gpu/loop.ts:662-685), the attempt is appended to the chunk's
+// graph.  The schedule depends on step counts only, so the host needs nothing
+// from the device to lay it out.
 static void run_group(md_group *g, int64_t steps) {
   if (steps <= 0) return;
   MD_HIP(hipSetDevice(g->device));
@@ -739,24 +726,48 @@ static void run_group(md_group *g, int64_t steps) {
   if (g->graph_stale) drop_graphs(g);
   g->graph_stale = false;
   MD_HIP(hipMemsetAsync(g->status, 0, sizeof(uint32_t), g->stream));
-  int64_t done = 0;
-  while (done < steps) {
-    const int chunk = (int)std::min<int64_t>(stride, steps - done);
-    if (chunk > 1)
-      MD_HIP(hipGraphLaunch(stride_graph(g, chunk, stride), g->stream));
-    else
-      launch_step(g);
-    uint32_t status = 0;
-    MD_HIP(hipMemcpyAsync(&status, g->status, sizeof status, hipMemcpyDeviceToHost, g->stream));
-    MD_HIP(hipStreamSynchronize(g->stream));
-    if (status) {
-      for (md_system *s : g->reps) check_control(s);
-      throw md_error(MD_ERR_HIP, "a replica reported an error that its control block no longer holds");
+  md_system *const baro = g->reps.size() == 1 && md_baro_frequency(g->reps[0]) > 0 ? g->reps[0] : nullptr;
+  const int frequency = baro ? md_baro_frequency(baro) : 0;
+  const int baro_step0 = baro ? md_baro_step(baro) : 0;
+  try {
+    int64_t done = 0;
+    while (done < steps) {
+      int chunk = (int)std::min<int64_t>(stride, steps - done);
+      bool attempt = false;
+      if (baro) {
+        int &since = md_baro_step(baro);
+        chunk = std::min(chunk, std::max(1, frequency - since));
+        since += chunk;
+        if (since >= frequency) {
+          since -= frequency;
+          attempt = true;
+        }
+      }
+      if (chunk > 1) {
+        MD_HIP(hipGraphLaunch(stride_graph(g, chunk, stride, attempt), g->stream));
+      } else {
+        launch_step(g);
+        if (attempt) md_baro_attempt(baro);
+      }
+      uint32_t status = 0;
+      MD_HIP(hipMemcpyAsync(&status, g->status, sizeof status, hipMemcpyDeviceToHost, g->stream));
+      MD_HIP(hipStreamSynchronize(g->stream));
+      if (status) {
+        for (md_system *s : g->reps) check_control(s);
+        throw md_error(MD_ERR_HIP, "a replica reported an error that its control block no longer holds");
+      }
+      done += chunk;
     }
-    done += chunk;
+    // an error raised in the last step is seen by no later prologue
+    for (md_system *s : g->reps) check_control(s);
+  } catch (...) {
+    if (baro) {
+      md_baro_step(baro) = baro_step0;
+      md_baro_sync_box(baro);
+    }
+    throw;
   }
-  // an error raised in the last step is seen by no later prologue
-  for (md_system *s : g->reps) check_control(s);
+  if (baro) md_baro_sync_box(baro);
   for (md_system *s : g->reps) s->steps_done += steps;
 }
 
@@ -792,25 +803,10 @@ static void evaluate(md_system *s, double *forces, double *groups, double *total
   }
   MD_HIP(hipStreamSynchronize(s->stream));
   check_control(s);
-  double g[MD_GROUP_COUNT] = {0};
-  g[MD_GROUP_BOND] = md_fixed_to_double(e[MD_E_BOND]);
-  g[MD_GROUP_ANGLE] = md_fixed_to_double(e[MD_E_ANGLE]);
-  g[MD_GROUP_TORSION] = md_fixed_to_double(e[MD_E_TORSION]);
-  g[MD_GROUP_RESTRAINT] = md_fixed_to_double(e[MD_E_RESTRAINT]);
-  if (s->has_nonbonded) {
-    double volume = box_volume(s->box);
-    g[MD_GROUP_DIRECT] = md_fixed_to_double(e[MD_E_PAIR]) + md_fixed_to_double(e[MD_E_EXCEPTION]) +
-                         (s->dispersion ? s->disp_coeff / volume : 0.0);
-    // self and net-charge terms (nonbonded.ts:594-617)
-    double self = -MD_ONE_4PI_EPS0 * s->alpha * s->q2_sum / std::sqrt(M_PI);
-    double net = -M_PI * s->q_total * s->q_total * MD_ONE_4PI_EPS0 / (2.0 * s->alpha * s->alpha * volume);
-    g[MD_GROUP_RECIPROCAL] = md_fixed_to_double(e[MD_E_RECIPROCAL]) + self + net;
-  }
+  double g[MD_GROUP_COUNT];
+  const double t = md_energy_groups(e, md_energy_constants(s), box_volume(s->box), g);
   if (groups) std::memcpy(groups, g, sizeof g);
-  if (total) {
-    *total = 0;
-    for (double v : g) *total += v;
-  }
+  if (total) *total = t;
   if (forces)
     for (int i = 0; i < s->n; ++i)
       for (int d = 0; d < 3; ++d) forces[3 * i + d] = md_fixed_to_double(f[(size_t)d * s->np + i]);
@@ -1222,6 +1218,7 @@ md_err md_set_box(md_system *s, const double box[9]) {
   MD_API_BEGIN
   if (!s || !box) throw md_error(MD_ERR_INVALID_ARGUMENT, "null argument");
   apply_box(s, box);
+  md_baro_set_box(s);
   MD_API_END
 }
 
@@ -1277,6 +1274,21 @@ md_err md_constrain_velocities(md_system *s, double tolerance) {
   md_constrain(s->self, 1, tolerance);
   MD_HIP(hipStreamSynchronize(s->stream));
   s->momentum_dirty = true;
+  MD_API_END
+}
+
+md_err md_barostat_attach(md_system *s, const md_barostat_desc *desc) {
+  MD_API_BEGIN
+  if (!s || !desc) throw md_error(MD_ERR_INVALID_ARGUMENT, "null argument");
+  md_baro_attach(s, desc);
+  MD_API_END
+}
+
+md_err md_barostat_get(md_system *s, md_barostat_state *state) {
+  MD_API_BEGIN
+  if (!s || !state) throw md_error(MD_ERR_INVALID_ARGUMENT, "null argument");
+  MD_HIP(hipSetDevice(s->device));
+  md_baro_state(s, state);
   MD_API_END
 }
 
@@ -1397,6 +1409,8 @@ md_err md_batch_create(md_system *const *systems, int32_t count, md_batch **out)
       throw md_error(MD_ERR_INVALID_ARGUMENT, "the systems of a batch must share one device");
     for (int q = 0; q < r; ++q)
       if (reps[q] == reps[r]) throw md_error(MD_ERR_INVALID_ARGUMENT, "a system appears twice in a batch");
+    if (md_baro_frequency(reps[r]) > 0)
+      throw md_error(MD_ERR_INVALID_ARGUMENT, "a system carrying a barostat cannot join a batch");
   }
   md_batch *b = new md_batch();
   try {

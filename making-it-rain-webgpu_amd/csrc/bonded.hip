@@ -237,6 +237,19 @@ void md_bonded_group_refresh(md_group *g) {
   MD_HIP(hipStreamSynchronize(g->stream));
 }
 
+// Replica 0's box from a box in device memory (a barostat move).
+__global__ void k_bonded_box(md_term_args *T, const double *box) {
+  md_box d{};
+  bool bad = false;
+  md_box_from_matrix(box, d, bad);
+  T[0].box = d;
+}
+
+void md_bonded_box_from_device(md_group *g, const double *box) {
+  k_bonded_box<<<1, 1, 0, g->stream>>>(bg(g).table, box);
+  MD_HIP(hipGetLastError());
+}
+
 void md_bonded_group_free(md_group *g) {
   delete &bg(g);
   g->bonded = nullptr;

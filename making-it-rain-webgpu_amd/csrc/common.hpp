@@ -64,16 +64,16 @@ __host__ __device__ inline float md_from_fixed(int64_t v) {
 // f64 reading on the device (exact below 2^21 in magnitude)
 __device__ inline double md_fixed_to_double_dev(int64_t v) { return (double)v * 2.3283064365386963e-10; }
 
-// exact f64 reading, host side
-inline double md_fixed_to_double(int64_t v) {
+// exact f64 reading (host, and device code that must agree with the host)
+__host__ __device__ inline double md_fixed_to_double(int64_t v) {
   int64_t whole = v / 4294967296LL;
   int64_t frac = v - whole * 4294967296LL;
   return (double)whole + (double)frac / 4294967296.0;
 }
 
-// f64 -> fixed, for host uploads of positions (round to nearest, exact for
-// values representable at 2^-32)
-inline int64_t md_double_to_fixed(double x) {
+// f64 -> fixed, for uploads of positions (round to nearest, ties to even;
+// exact for values representable at 2^-32)
+__host__ __device__ inline int64_t md_double_to_fixed(double x) {
   double s = x * 4294967296.0;
   return (int64_t)llrint(s);
 }
@@ -121,6 +121,67 @@ struct md_box {
   double invd[9];
   int rectangular;
 };
+
+// Box arithmetic shared by the host and the device.  Contraction into fused
+// multiply-adds is off in these helpers, so both compute the same doubles.
+//
+// The box constants of a reduced box b (rows a, b, c; 9 doubles).  `bad` is set when a length lies outside the
+// fixed-point range (|x| >= 2^31 nm, or not finite).
+__host__ __device__ inline void md_box_from_matrix(const double *b, md_box &d, bool &bad) {
+#pragma clang fp contract(off)
+  for (int k = 0; k < 9; ++k)
+    if (!(fabs(b[k]) < 2147483648.0)) bad = true;
+  for (int k = 0; k < 3; ++k) {
+    d.fa[k] = md_double_to_fixed(b[k]);
+    d.fb[k] = md_double_to_fixed(b[3 + k]);
+    d.fc[k] = md_double_to_fixed(b[6 + k]);
+    d.a[k] = (float)b[k];
+    d.b[k] = (float)b[3 + k];
+    d.c[k] = (float)b[6 + k];
+  }
+  const double ax = b[0], bx = b[3], by = b[4], cx = b[6], cy = b[7], cz = b[8];
+  const double m[9] = {1.0 / ax, 0, 0, -bx / (ax * by), 1.0 / by, 0, ((bx * cy) / (by * cz) - cx / cz) / ax,
+                       -cy / (by * cz), 1.0 / cz};
+  for (int k = 0; k < 9; ++k) {
+    d.invd[k] = m[k];
+    d.inv[k] = (float)m[k];
+  }
+  d.rectangular = bx == 0 && cx == 0 && cy == 0;
+}
+
+__host__ __device__ inline double md_box_volume(const double *b) { return b[0] * b[4] * b[8]; }
+
+// perpendicular widths of the cell along a, b and c
+__host__ __device__ inline void md_box_widths(const double *b, double w[3]) {
+#pragma clang fp contract(off)
+  const double v = md_box_volume(b);
+  const double *u[3] = {b + 3, b + 6, b};
+  const double *t[3] = {b + 6, b, b + 3};
+  for (int k = 0; k < 3; ++k) {
+    const double *p = u[k], *q = t[k];
+    const double x = p[1] * q[2] - p[2] * q[1], y = p[2] * q[0] - p[0] * q[2], z = p[0] * q[1] - p[1] * q[0];
+    w[k] = v / sqrt(x * x + y * y + z * z);
+  }
+}
+
+// ties to even, as reduceBox (src/reference/triclinic.ts:93-125)
+__host__ __device__ inline double md_reduce_by(double ratio) {
+  const double nearest = floor(ratio + 0.5);
+  if (fabs(ratio - trunc(ratio)) == 0.5 && fmod(nearest, 2.0) != 0.0) return nearest - (ratio > 0 ? 1.0 : -1.0);
+  return nearest;
+}
+
+// OpenMM's reduced form of a lower-triangular box, in place
+__host__ __device__ inline void md_reduce_box(double *v) {
+#pragma clang fp contract(off)
+  double t = md_reduce_by(v[7] / v[4]);
+  for (int k = 0; k < 3; ++k) v[6 + k] -= t * v[3 + k];
+  t = md_reduce_by(v[6] / v[0]);
+  for (int k = 0; k < 3; ++k) v[6 + k] -= t * v[k];
+  t = md_reduce_by(v[3] / v[0]);
+  for (int k = 0; k < 3; ++k) v[3 + k] -= t * v[k];
+  v[1] = v[2] = v[5] = 0.0;
+}
 
 // Nearest image of a fixed-point difference: integer box counts picked in
 // f32, subtracted exactly in fixed point, against c, then b, then a -- the

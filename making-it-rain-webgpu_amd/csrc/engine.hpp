@@ -53,7 +53,37 @@ struct md_control {
   int64_t shake_passes;            // SHAKE update passes over all cluster-steps
   int64_t shake_clusters;          // cluster-steps SHAKE has run
   int64_t pair_blocks;             // 8x8 cluster-pair blocks evaluated, when counted
+  uint32_t barostat_error;         // a barostat move failed: 1 the trial box is too small for the
+                                   // cutoff, 2 a scaled position left the fixed-point range
 };
+
+// The per-group energies of the energy slots and the host constants
+// (dispersion, self and net-charge terms), and their total -- the same on the
+// host (md_evaluate) and on the device (the barostat's Metropolis test).
+struct md_energy_consts {
+  int has_nonbonded, dispersion;
+  double disp_coeff, alpha, q_total, q2_sum;
+};
+__host__ __device__ inline double md_energy_groups(const int64_t *e, const md_energy_consts &k, double volume,
+                                                   double *g) {
+#pragma clang fp contract(off)
+  for (int i = 0; i < MD_GROUP_COUNT; ++i) g[i] = 0.0;
+  g[MD_GROUP_BOND] = md_fixed_to_double(e[MD_E_BOND]);
+  g[MD_GROUP_ANGLE] = md_fixed_to_double(e[MD_E_ANGLE]);
+  g[MD_GROUP_TORSION] = md_fixed_to_double(e[MD_E_TORSION]);
+  g[MD_GROUP_RESTRAINT] = md_fixed_to_double(e[MD_E_RESTRAINT]);
+  if (k.has_nonbonded) {
+    g[MD_GROUP_DIRECT] = md_fixed_to_double(e[MD_E_PAIR]) + md_fixed_to_double(e[MD_E_EXCEPTION]) +
+                         (k.dispersion ? k.disp_coeff / volume : 0.0);
+    // self and net-charge terms (nonbonded.ts:594-617)
+    const double self = -MD_ONE_4PI_EPS0 * k.alpha * k.q2_sum / sqrt(M_PI);
+    const double net = -M_PI * k.q_total * k.q_total * MD_ONE_4PI_EPS0 / (2.0 * k.alpha * k.alpha * volume);
+    g[MD_GROUP_RECIPROCAL] = md_fixed_to_double(e[MD_E_RECIPROCAL]) + self + net;
+  }
+  double total = 0.0;
+  for (int i = 0; i < MD_GROUP_COUNT; ++i) total += g[i];
+  return total;
+}
 
 struct md_nb_params {
   int nclusters, nslots;  // 8-atom clusters; slots in whole 64-atom super-clusters
@@ -94,6 +124,7 @@ struct md_system;
 struct md_ws_net;
 struct md_warmstart;
 struct md_min_work;
+struct md_baro;
 
 // Host side of a block map: per-replica block counts -> device prefix.
 struct md_launch {
@@ -110,13 +141,15 @@ struct md_group {
   int max_cluster_atoms = 0;
   // per-subsystem launch tables (each file's private struct)
   void *nb = nullptr, *bonded = nullptr, *pme = nullptr, *integ = nullptr;
-  // captured strides: a whole stride and the last shorter remainder run
+  // captured strides: a whole stride and the last shorter remainder run, each
+  // with or without a barostat attempt at its end
   struct captured {
     int steps = 0;
+    bool attempt = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
   };
-  captured graphs[2];
+  captured graphs[4];
   int max_batch = 0;  // steps per captured graph (0: the default)
   // set by each step's prologue when a replica's control block holds an
   // error (list or frame overflow, diverged constraints); read after every
@@ -230,6 +263,9 @@ struct md_system {
   float *rec_x = nullptr, *rec_t = nullptr;  // sample ring of a recording run
   int64_t rec_first_step = 0, rec_cap = 0;
 
+  // ---- the device barostat, when attached ------------------------------------
+  md_baro *baro = nullptr;
+
   // ---- the minimiser's work vectors, made on the first md_minimise ----------
   md_min_work *min_work = nullptr;
   std::vector<int> con_p1, con_p2;    // the constraints (host copies)
@@ -298,6 +334,23 @@ void md_step_prologue(md_group *g, bool advance);
 void md_integrate(md_group *g);
 void md_momentum(md_group *g);
 void md_constrain(md_group *g, int what, double tolerance);  // 0 positions, 1 velocities
+
+// ---- box changes made on the device (the barostat): each subsystem rewrites its
+// table entry (replica 0 of the group) from a reduced box in device memory -------
+void md_nb_box_from_device(md_group *g, const double *box);
+void md_bonded_box_from_device(md_group *g, const double *box);
+void md_pme_box_from_device(md_group *g, const double *box);
+
+// ---- the device barostat (barostat.hip) ---------------------------------------
+md_energy_consts md_energy_constants(const md_system *s);
+void md_baro_attach(md_system *s, const md_barostat_desc *d);
+void md_baro_attempt(md_system *s);        // enqueue one attempt on the system's own group
+void md_baro_set_box(md_system *s);         // the host's box onto the device state
+void md_baro_sync_box(md_system *s);        // the device state's box into s->box (waits)
+void md_baro_state(md_system *s, md_barostat_state *out);
+void md_baro_release(md_system *s);
+int md_baro_frequency(const md_system *s);  // 0 without a barostat
+int &md_baro_step(md_system *s);            // steps since the last attempt
 
 // ---- the device-resident minimiser (minimise.hip) -----------------------------
 // L-BFGS work vectors: MD_MIN_VECTORS f64 3N-vectors in planes [3][np]

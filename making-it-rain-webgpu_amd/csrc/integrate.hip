@@ -867,7 +867,7 @@ __global__ void k_step_prologue(const md_int_args *__restrict__ T, int R, int ad
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= R) return;
   md_control *ctl = T[r].ctl;
-  if (ctl->overflow | ctl->diverged) atomicOr(T[r].group_status, 1u);
+  if (ctl->overflow | ctl->diverged | ctl->barostat_error) atomicOr(T[r].group_status, 1u);
   if (advance) {
     ctl->step_current = ctl->step_next;
     ctl->step_next = ctl->step_next + 1;

@@ -87,6 +87,11 @@ struct md_build_ctx {
 // wavefronts per workgroup (a replica that did not rebuild dispatches
 // MD_MASK_WAVES / MD_MASK_GROUP workgroups per super-cluster that return at once)
 #define MD_MASK_WAVES 16
+// entry meta word: bits 0-7 the i-clusters of S that meet J, bits 8-15 those
+// that are J itself at the zero image, and MD_META_CLEAN when no active block
+// has an excluded, padding or lower-triangle pair (its masks are all ones and
+// the pair kernel does not read them)
+#define MD_META_CLEAN (1 << 16)
 #define MD_MASK_GROUP 4
 
 struct md_list_ptrs {
@@ -737,6 +742,8 @@ __global__ void __launch_bounds__(64 * MD_MASK_GROUP) k_nb_masks(const md_nb_arg
     const int meta = L.entry_imask[e];
     if (lane == 0) atomicAdd(&L.j_count[J], 1);
     const int ja = slot_atom[J * MD_CLUSTER + jl];
+    bool clean = true;
+    uint64_t masks[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       bool ok = ((meta >> q) & 1) && ia[q] >= 0 && ja >= 0;
@@ -754,7 +761,18 @@ __global__ void __launch_bounds__(64 * MD_MASK_GROUP) k_nb_masks(const md_nb_arg
         }
       }
       const uint64_t m = __ballot(ok);
-      if (lane == 0) L.entry_excl[(size_t)e * 8 + q] = m;
+      masks[q] = m;
+      if ((meta >> q) & 1) clean = clean && m == ~0ull;
+    }
+    // a clean entry's masks are never read: only its meta word changes
+    if (clean) {
+      if (lane == 0) L.entry_imask[e] = meta | MD_META_CLEAN;
+    } else if (lane < 8) {
+      uint64_t m = masks[0];
+#pragma unroll
+      for (int q = 1; q < 8; ++q)
+        if (lane == q) m = masks[q];
+      L.entry_excl[(size_t)e * 8 + lane] = m;
     }
   }
 }
@@ -1053,7 +1071,7 @@ __device__ __forceinline__ void md_pair_segment(const md_pair_args &A, int seg, 
         if ((active >> q) & 1) {
           // bit l of the mask allows lane l's pair: with the in-cutoff ballot
           // it is the block's lane mask of counted pairs
-          const uint64_t allowed = md_uniform(ex + q);
+          const uint64_t allowed = (meta & MD_META_CLEAN) ? ~0ull : md_uniform(ex + q);
           float dx = (float)(int)(xj - (unsigned)xi[q]), dy = (float)(int)(yj - (unsigned)yi[q]),
                 dz = (float)(int)(zj - (unsigned)zi[q]);
           float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
@@ -1380,6 +1398,28 @@ void md_nb_free(md_system *s) {
   s->nb_ext = nullptr;
 }
 
+// The list context's box-dependent fields, from a reduced box b and its
+// constants: the same on the host (table set-up) and on the device (a box
+// changed by the Monte Carlo barostat).
+__host__ __device__ inline void md_nb_box_fields(md_build_ctx &c, const double *b, const md_box &d) {
+#pragma clang fp contract(off)
+  double w[3];
+  md_box_widths(b, w);
+  // half the narrowest perpendicular width of the current box
+  c.half_min_width = (float)(0.5 * fmin(w[0], fmin(w[1], w[2])));
+  // md_min_image_fixed rounds along c, then b, then a on Cartesian components,
+  // which leaves the fractional coordinates of the reduced difference within
+  // fc = 1/2, fb = 1/2 + fc |cy|/by and fa = 1/2 + (fb |bx| + fc |cx|)/ax (plus
+  // a margin for the f32 rounding); a rectangular box gives 1/2 on every axis
+  const double fc = 0.5, fb = 0.5 + fc * fabs(b[7]) / b[4], fa = 0.5 + (fb * fabs(b[3]) + fc * fabs(b[6])) / b[0];
+  const double f[3] = {fa, fb, fc};
+  for (int k = 0; k < 3; ++k) {
+    c.width[k] = (float)(w[k] * (1.0 - 1e-5));
+    c.fmax[k] = (float)(f[k] + 1e-5);
+  }
+  c.box = d;
+}
+
 static md_build_ctx build_ctx(md_system *s) {
   auto &B = nbuf(s);
   md_build_ctx c{};
@@ -1396,27 +1436,7 @@ static md_build_ctx build_ctx(md_system *s) {
   c.segment_capacity = B.segment_capacity;
   c.list_cutoff2 = s->nb.list_cutoff2;
   c.list_cutoff = B.list_cutoff;
-  // half the narrowest perpendicular width of the current box
-  const double *b = s->box;
-  auto cross_norm = [](const double *u, const double *w) {
-    double x = u[1] * w[2] - u[2] * w[1], y = u[2] * w[0] - u[0] * w[2], z = u[0] * w[1] - u[1] * w[0];
-    return std::sqrt(x * x + y * y + z * z);
-  };
-  double v = b[0] * b[4] * b[8];
-  const double wa = v / cross_norm(b + 3, b + 6), wb = v / cross_norm(b + 6, b), wc = v / cross_norm(b, b + 3);
-  c.half_min_width = (float)(0.5 * std::min(wa, std::min(wb, wc)));
-  // md_min_image_fixed rounds along c, then b, then a on Cartesian components,
-  // which leaves the fractional coordinates of the reduced difference within
-  // fc = 1/2, fb = 1/2 + fc |cy|/by and fa = 1/2 + (fb |bx| + fc |cx|)/ax (plus
-  // a margin for the f32 rounding); a rectangular box gives 1/2 on every axis
-  const double fc = 0.5, fb = 0.5 + fc * std::fabs(b[7]) / b[4],
-               fa = 0.5 + (fb * std::fabs(b[3]) + fc * std::fabs(b[6])) / b[0];
-  const double w[3] = {wa, wb, wc}, f[3] = {fa, fb, fc};
-  for (int d = 0; d < 3; ++d) {
-    c.width[d] = (float)(w[d] * (1.0 - 1e-5));
-    c.fmax[d] = (float)(f[d] + 1e-5);
-  }
-  c.box = s->dbox;
+  md_nb_box_fields(c, s->box, s->dbox);
   return c;
 }
 
@@ -1576,6 +1596,20 @@ void md_nb_group_refresh(md_group *g) {
   MD_HIP(hipMemcpyAsync(nbg(g).table, host.data(), sizeof(md_nb_args) * host.size(), hipMemcpyHostToDevice,
                         g->stream));
   MD_HIP(hipStreamSynchronize(g->stream));
+}
+
+// The list context of replica 0 from a box in device memory (a barostat move):
+// the values md_nb_group_refresh would upload for that box.
+__global__ void k_nb_box(md_nb_args *T, const double *box) {
+  md_box d{};
+  bool bad = false;
+  md_box_from_matrix(box, d, bad);
+  md_nb_box_fields(T[0].c, box, d);
+}
+
+void md_nb_box_from_device(md_group *g, const double *box) {
+  k_nb_box<<<1, 1, 0, g->stream>>>(nbg(g).table, box);
+  MD_HIP(hipGetLastError());
 }
 
 void md_nb_group_free(md_group *g) {

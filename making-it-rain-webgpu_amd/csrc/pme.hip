@@ -593,6 +593,18 @@ void md_pme_free(md_system *s) {
   s->pme_ext = nullptr;
 }
 
+// The mesh's box-dependent arguments, from a reduced box b and its
+// constants: the same on the host (table set-up) and on the device (a box
+// changed by the Monte Carlo barostat).
+__host__ __device__ inline void md_pme_box_fields(md_pme_args &P, const double *b, const md_box &d) {
+#pragma clang fp contract(off)
+  for (int k = 0; k < 9; ++k) {
+    P.inv[k] = d.invd[k];
+    P.invf[k] = d.inv[k];
+  }
+  P.prefactor = (float)(MD_ONE_4PI_EPS0 * 2.0 * M_PI / md_box_volume(b));
+}
+
 static md_pme_args pme_args(md_system *s) {
   auto &H = pme_host(s);
   md_pme_args P;
@@ -607,12 +619,7 @@ static md_pme_args pme_args(md_system *s) {
   P.mod_x = H.mod;
   P.mod_y = H.mod + P.nx;
   P.mod_z = H.mod + P.nx + P.ny;
-  for (int k = 0; k < 9; ++k) {
-    P.inv[k] = s->dbox.invd[k];
-    P.invf[k] = s->dbox.inv[k];
-  }
-  double volume = s->box[0] * s->box[4] * s->box[8];
-  P.prefactor = (float)(MD_ONE_4PI_EPS0 * 2.0 * M_PI / volume);
+  md_pme_box_fields(P, s->box, s->dbox);
   P.exp_scale = (float)(-1.0 / (4.0 * s->alpha * s->alpha));
   return P;
 }
@@ -684,6 +691,20 @@ void md_pme_group_refresh(md_group *g) {
   MD_HIP(hipMemcpyAsync(pg(g).table, host.data(), sizeof(md_pme_table) * host.size(), hipMemcpyHostToDevice,
                         g->stream));
   MD_HIP(hipStreamSynchronize(g->stream));
+}
+
+// Replica 0's box-dependent mesh arguments from a box in device memory (a
+// barostat move).
+__global__ void k_pme_box(md_pme_table *T, const double *box) {
+  md_box d{};
+  bool bad = false;
+  md_box_from_matrix(box, d, bad);
+  md_pme_box_fields(T[0].P, box, d);
+}
+
+void md_pme_box_from_device(md_group *g, const double *box) {
+  k_pme_box<<<1, 1, 0, g->stream>>>(pg(g).table, box);
+  MD_HIP(hipGetLastError());
 }
 
 void md_pme_group_free(md_group *g) {

@@ -44,6 +44,9 @@ class RunOptions:
     mesh: Optional[Sequence[int]] = None
     device: int = 0
     maxBatch: int = 0  # steps per captured HIP graph; 0 = 50, 1 = no graphs
+    # where a MonteCarloBarostat's moves run: "device" (inside the captured
+    # strides, md_barostat_attach) or "host" (between strides, as upstream)
+    barostat: str = "device"
 
 
 @dataclass
@@ -128,14 +131,26 @@ class HipEngine:
         cmm = system.force("CMMotionRemover")
         self.degreesOfFreedom = int(3 * np.count_nonzero(self._moving) - system.constraintCount
                                     - (3 if cmm is not None else 0))
-        # Monte Carlo barostat (src/gpu/loop.ts:602-685): host-driven moves
-        # between strides, deviates from their own Threefry domain
+        # Monte Carlo barostat (src/gpu/loop.ts:602-685), deviates from their
+        # own Threefry domain: on the device inside the strides, or host-driven
+        # moves between strides
         self.barostat = system.force("MonteCarloBarostat")
         self.barostatState = baro.BarostatState.initial(self._box) if self.barostat is not None else None
         self.barostatDraw = 0
         self.barostatAttempts = 0
         self.barostatAccepted = 0
         self._molecules = None
+        self.lastBarostat = None
+        if o.barostat not in ("device", "host"):
+            raise ValueError(f"barostat must be 'device' or 'host', not {o.barostat!r}")
+        self._device_barostat = (self.barostat is not None and self.barostat.frequency > 0
+                                 and o.barostat == "device")
+        if self._device_barostat:
+            self._molecules = baro.find_molecules(system)
+            b = self.barostat
+            self._native.attach_barostat(b.defaultPressure, b.defaultTemperature, b.frequency,
+                                         self._molecules.starts, self._molecules.atoms, seed=o.seed,
+                                         volume_scale=self.barostatState.volumeScale)
 
     # -- ProtocolEngine ----------------------------------------------------
     @property
@@ -185,6 +200,12 @@ class HipEngine:
         if b is None or b.frequency <= 0:
             self._native.run(steps)
             return
+        if self._device_barostat:
+            try:
+                self._native.run(steps)
+            finally:
+                self._read_device_barostat()
+            return
         done = 0
         while done < steps:
             until = b.frequency - self.barostatState.step
@@ -192,6 +213,20 @@ class HipEngine:
             self._native.run(chunk)
             done += chunk
             self._attempt_barostat(chunk)
+
+    def _read_device_barostat(self) -> None:
+        """The device barostat's counters and box into the host mirror."""
+        st = self._native.barostat_state()
+        self._box = reduceBox(self._native.box())
+        self.barostatDraw = int(st["draws"])
+        self.barostatAttempts = int(st["attempts"])
+        self.barostatAccepted = int(st["accepted"])
+        s = self.barostatState
+        s.step = int(st["steps_since_attempt"])
+        s.volumeScale = float(st["volume_scale"])
+        s.attempted = int(st["window_attempted"])
+        s.accepted = int(st["window_accepted"])
+        self.lastBarostat = st
 
     def _attempt_barostat(self, steps_just_run: int) -> None:
         if not baro.acts_after_steps(self.barostatState, self.barostat.frequency, steps_just_run):
@@ -219,6 +254,9 @@ class HipEngine:
         self.barostatAttempts += 1
         if res.accepted:
             self.barostatAccepted += 1
+        self.lastBarostat = {"last_accepted": int(res.accepted), "last_energy_before": res.initialEnergy,
+                             "last_energy_after": res.finalEnergy, "last_work": res.work,
+                             "last_length_scale": res.lengthScale, "draws": self.barostatDraw}
 
     def sample(self) -> dict:
         r = self.report()

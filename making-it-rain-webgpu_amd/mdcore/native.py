@@ -120,6 +120,22 @@ class MinimiseResultC(C.Structure):
                 ("max_constraint_error", C.c_double)]
 
 
+class BarostatDescC(C.Structure):
+    _fields_ = [("pressure", C.c_double), ("temperature", C.c_double), ("frequency", C.c_int32),
+                ("n_molecules", C.c_int32), ("molecule_start", C.POINTER(C.c_int32)),
+                ("molecule_atoms", C.POINTER(C.c_int32)), ("seed", C.c_uint32), ("volume_scale", C.c_double),
+                ("draws", C.c_int64), ("steps_since_attempt", C.c_int32), ("window_attempted", C.c_int32),
+                ("window_accepted", C.c_int32)]
+
+
+class BarostatStateC(C.Structure):
+    _fields_ = [("attempts", C.c_int64), ("accepted", C.c_int64), ("draws", C.c_int64),
+                ("steps_since_attempt", C.c_int32), ("window_attempted", C.c_int32),
+                ("window_accepted", C.c_int32), ("volume_scale", C.c_double), ("last_accepted", C.c_int32),
+                ("last_energy_before", C.c_double), ("last_energy_after", C.c_double), ("last_work", C.c_double),
+                ("last_length_scale", C.c_double)]
+
+
 class ListFlags(C.Structure):
     _fields_ = [("rebuilds", C.c_int64), ("tiles", C.c_int64), ("tile_capacity", C.c_int64),
                 ("worst_displacement", C.c_double), ("overflowed", C.c_int32)]
@@ -169,6 +185,8 @@ def library():
         L.md_autotune.argtypes = [C.c_void_p, C.c_int32, C.POINTER(Schedule)]
         L.md_minimise.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.POINTER(MinimiseResultC)]
         L.md_batch_autotune.argtypes = [C.c_void_p, C.c_int32, C.POINTER(Schedule)]
+        L.md_barostat_attach.argtypes = [C.c_void_p, C.POINTER(BarostatDescC)]
+        L.md_barostat_get.argtypes = [C.c_void_p, C.POINTER(BarostatStateC)]
         _fp = P(C.c_float)
         L.md_warmstart_create.argtypes = [C.c_int32, C.c_uint32, C.c_int64, P(C.c_void_p)]
         L.md_warmstart_destroy.argtypes = [C.c_void_p]
@@ -390,6 +408,24 @@ class NativeSystem:
         """md_minimise: L-BFGS with constraint springs, searched on the device."""
         out = MinimiseResultC()
         check(library().md_minimise(self.handle, int(max_iterations), float(tolerance), C.byref(out)))
+        return {k: getattr(out, k) for k, _ in out._fields_}
+
+    def attach_barostat(self, pressure: float, temperature: float, frequency: int, molecule_start,
+                        molecule_atoms, seed: int = 0, volume_scale: float = 0.0, draws: int = 0,
+                        steps_since_attempt: int = 0, window_attempted: int = 0, window_accepted: int = 0):
+        """md_barostat_attach: the Monte Carlo barostat run on the device inside
+        ``run``; ``frequency`` 0 detaches it."""
+        start = np.ascontiguousarray(molecule_start, dtype=np.int32)
+        atoms = np.ascontiguousarray(molecule_atoms, dtype=np.int32)
+        d = BarostatDescC(float(pressure), float(temperature), int(frequency), int(start.shape[0] - 1),
+                          start.ctypes.data_as(_ip), atoms.ctypes.data_as(_ip), int(seed) & 0xFFFFFFFF,
+                          float(volume_scale), int(draws), int(steps_since_attempt), int(window_attempted),
+                          int(window_accepted))
+        check(library().md_barostat_attach(self.handle, C.byref(d)))
+
+    def barostat_state(self) -> dict:
+        out = BarostatStateC()
+        check(library().md_barostat_get(self.handle, C.byref(out)))
         return {k: getattr(out, k) for k, _ in out._fields_}
 
     def constrain_positions(self, tolerance: float):

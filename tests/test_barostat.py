@@ -134,3 +134,18 @@ def test_attempt_on_the_oracle_potential(small):
                                      + 6.02214076e23 * 1e-25 * res.deltaVolume
                                      - m.count * kT * math.log((v + res.deltaVolume) / v), rel=1e-12)
     assert res.initialEnergy == pytest.approx(potential(small.state.positions, box), rel=1e-14)
+
+
+def test_molecule_table_is_what_the_device_barostat_takes(small):
+    """md_barostat_attach takes molecules as a CSR table: starts from 0 to
+    N, every particle exactly once, ascending within each molecule (the
+    order the device sums a centroid in, as numpy's bincount does)."""
+    mol = baro.find_molecules(small.system)
+    n = small.system.particleCount
+    assert mol.starts[0] == 0 and mol.starts[-1] == n
+    assert np.all(np.diff(mol.starts) > 0)
+    assert np.array_equal(np.sort(mol.atoms), np.arange(n))
+    for m in range(mol.count):
+        block = mol.atoms[mol.starts[m]:mol.starts[m + 1]]
+        assert np.all(np.diff(block) > 0)
+        assert np.all(mol.moleculeOf[block] == m)

@@ -102,3 +102,97 @@ def test_npt_run_attempts_on_schedule_and_keeps_constraints(npt):
     assert rel.max() < 2e-6
     assert np.all(np.isfinite(x))
     assert e.barostatAccepted > 0
+
+
+def _twin(b, e, **kw):
+    """A second engine at e's state (positions, velocities, step, box)."""
+    from mdcore.state import OpenMMState
+    st = OpenMMState(positions=e.positions(), velocities=e.velocities(), boxVectors=e.boxVectors(),
+                     stepCount=e.steps)
+    return HipEngine(b.system, st, RunOptions(seed=4, **kw))
+
+
+def test_device_attempt_matches_the_host_attempt(npt):
+    """One attempt on the device (inside the captured stride) and one on the
+    host (upstream's path), from the same state: the same deviates, the same
+    decision, the same box and positions.  The device's cube root, logarithm
+    and exponential are its own library's, so the box and the positions may
+    differ by the last bit; everything before them is the same arithmetic."""
+    b, e = npt
+    dev = _twin(b, e, barostat="device")
+    host = _twin(b, e, barostat="host")
+    for k in range(3):
+        dev.run(25)
+        host.run(25)
+        d, h = dev.lastBarostat, host.lastBarostat
+        xd, xh = dev.positions(), host.positions()
+        bd, bh = np.asarray(dev.boxVectors()), np.asarray(host.boxVectors())
+        print("device", d["last_accepted"], d["last_energy_before"], d["last_energy_after"], d["last_work"],
+              "host", h["last_accepted"], h["last_energy_before"], h["last_energy_after"], h["last_work"],
+              "box diff", np.max(np.abs(bd - bh)), "position diff", np.max(np.abs(xd - xh)))
+        assert d["last_accepted"] == h["last_accepted"]
+        assert d["draws"] == h["draws"]
+        assert abs(d["last_work"] - h["last_work"]) < 1e-6 * abs(h["last_energy_before"])
+        if k == 0:
+            # the same state and arithmetic up to the library functions
+            assert abs(d["last_energy_before"] - h["last_energy_before"]) < 1e-12 * abs(h["last_energy_before"])
+            np.testing.assert_allclose(bd, bh, rtol=1e-14, atol=0)
+            assert np.max(np.abs(xd - xh)) < 1e-8
+        else:
+            # a last-bit difference in a box length moves f32 forces by their
+            # last bit, which 25 steps of dynamics grow but do not amplify far
+            assert abs(d["last_energy_before"] - h["last_energy_before"]) < 1e-6 * abs(h["last_energy_before"])
+            np.testing.assert_allclose(bd, bh, rtol=1e-10, atol=0)
+            assert np.max(np.abs(xd - xh)) < 1e-5
+    assert dev.barostatAttempts == host.barostatAttempts == 3
+    dev.destroy()
+    host.destroy()
+
+
+def test_device_barostat_graphs_equal_direct_launches(npt):
+    """The attempt inside captured strides and launched kernel by kernel (no
+    graphs) gives the same bits, over runs whose lengths cut the schedule
+    at different places."""
+    b, e = npt
+    g = _twin(b, e)
+    d = _twin(b, e, maxBatch=1)
+    for n in (30, 45, 25):
+        g.run(n)
+        d.run(n)
+    assert g.barostatAttempts == d.barostatAttempts == 4
+    assert np.array_equal(g.positions(), d.positions())
+    assert np.array_equal(np.asarray(g.boxVectors()), np.asarray(d.boxVectors()))
+    assert g.barostatState.step == d.barostatState.step == 0
+    g.destroy()
+    d.destroy()
+
+
+def test_device_barostat_leaves_tables_matching_its_box(npt):
+    """After device attempts have changed the box, the forces at the final
+    state match the oracle's at the box the engine reports (every subsystem's
+    box constants were rewritten on the device), and the constraints hold."""
+    b, e = npt
+    t = _twin(b, e)
+    start = np.asarray(t.boxVectors())
+    t.run(200)
+    box = np.asarray(t.boxVectors())
+    x = t.positions()
+    _, groups, f = t._native.evaluate(True)
+    o = O.OracleSystem(b.system)
+    _, og, of = o.potential(x, box, grid=t.mesh)
+    err, _ = edge.rel_rms_edge(f, of, b.system, x, box, o.alpha())
+    gerr = [abs(groups[k] - og[k]) / max(abs(og[k]), 1.0) for k in range(6)]
+    s = b.system
+    xr = x.reshape(-1, 3)
+    r = np.linalg.norm(xr[s.constraintP1] - xr[s.constraintP2], axis=1)
+    rel = np.abs(r - s.constraintDistance) / s.constraintDistance
+    print("device barostat: attempts", t.barostatAttempts, "accepted", t.barostatAccepted, "volume",
+          np.prod(np.diag(start)), "->", np.prod(np.diag(box)), "force rel rms", err, "groups", gerr,
+          "constraints", rel.max())
+    assert t.barostatAttempts == 8
+    assert t.barostatAccepted > 0
+    assert not np.array_equal(box, start)
+    assert err < 2e-6
+    assert max(gerr) < 1e-5
+    assert rel.max() < 2e-6
+    t.destroy()

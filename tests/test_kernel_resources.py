@@ -22,7 +22,7 @@ HIPCC = "/opt/rocm/bin/hipcc"
 FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
          "-packed-fp32-ops", "--cuda-device-only", "-S"]
 SOURCES = ["api.hip", "nonbonded.hip", "bonded.hip", "pme.hip", "integrate.hip", "warmstart.hip", "minimise.hip",
-           "noalloc.hip"]
+           "noalloc.hip", "barostat.hip"]
 
 
 def _report(src, out_dir):
