@@ -205,6 +205,18 @@ __device__ inline void md_min_image_fixed(int64_t &dx, int64_t &dy, int64_t &dz,
   if (ka != 0.0f) dx -= (int64_t)ka * bx.fa[0];
 }
 
+// ---- XCD-aware block order ----------------------------------------------------
+//
+// An MI355X dispatches a launch's workgroups to its 8 XCDs in turn (block b to
+// XCD b mod 8), and each XCD has its own L2.  md_xcd_block(b, total) gives XCD
+// k one contiguous run of logical blocks instead of every 8th, so blocks that
+// read neighbouring data (atoms in spatial order reading the same mesh
+// region) share an L2.  A bijection on [0, total) for any total.
+__device__ __forceinline__ int md_xcd_block(int b, int total) {
+  const int q = total >> 3, r = total & 7, k = b & 7;
+  return k * q + min(k, r) + (b >> 3);
+}
+
 // ---- wave-level integer reduction (wavefront of 64) -------------------------
 
 __device__ inline int64_t md_wave_sum_i64(int64_t v) {

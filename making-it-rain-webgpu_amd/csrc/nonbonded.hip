@@ -838,9 +838,11 @@ __global__ void k_nb_pack(const md_nb_args *__restrict__ T, md_bmap map, int *se
 // and the slot cleared for the next step.  Kernels on one stream run in order
 // and a slot is the only one holding its atom, so a plain read-add-write is
 // enough.
+// Blocks laid out XCD by XCD: neighbouring clusters read j-force records of
+// neighbouring entries, so they share one L2.
 __global__ void k_nb_forces(const md_nb_args *__restrict__ T, md_bmap map) {
   int blk;
-  const md_nb_args &A = md_nb_entry(T, map, blk);
+  const md_nb_args &A = T[md_locate(map, md_xcd_block(blockIdx.x, gridDim.x), blk)];
   const int nslots = A.c.nslots, np = A.c.np;
   const int s = blk * blockDim.x + threadIdx.x;
   if (s >= nslots) return;
@@ -855,10 +857,10 @@ __global__ void k_nb_forces(const md_nb_args *__restrict__ T, md_bmap map) {
   const int J = s / MD_CLUSTER, q = s - J * MD_CLUSTER;
   const float *jf = A.P.j_force;
   for (int k = A.L.j_start[J], end = A.L.j_start[J + 1]; k < end; ++k) {
-    const float *e = jf + (size_t)A.L.j_entries[k] * 24 + q;
+    const float *e = jf + (size_t)A.L.j_entries[k] * 24 + q * 3;
     f[0] += md_to_fixed(e[0]);
-    f[1] += md_to_fixed(e[8]);
-    f[2] += md_to_fixed(e[16]);
+    f[1] += md_to_fixed(e[1]);
+    f[2] += md_to_fixed(e[2]);
   }
   for (int d = 0; d < 3; ++d) A.force[d * np + a] += f[d];
 }
@@ -1094,8 +1096,14 @@ __device__ __forceinline__ void md_pair_segment(const md_pair_args &A, int seg, 
       gy = md_sum_same_j(gy);
       gz = md_sum_same_j(gz);
     }
-    // lanes 0-23 hold (x, y, z) of j atoms 0-7: one 96-byte store
-    if (lane < 24) A.j_force[(size_t)e * 24 + lane] = lane < 8 ? gx : (lane < 16 ? gy : gz);
+    // lanes 0-7 hold (x, y, z) of j atoms 0-7: one 96-byte record, j-major,
+    // so the slot thread that adds j's share reads 12 contiguous bytes
+    if (lane < 8) {
+      float *r = A.j_force + (size_t)e * 24 + lane * 3;
+      r[0] = gx;
+      r[1] = gy;
+      r[2] = gz;
+    }
     if (e + 1 == unit_end) {  // wave-uniform: the end of a unit
       // i forces: sum the 8 lanes that share i (same il), fixed order; lane
       // jl keeps q = jl's sum

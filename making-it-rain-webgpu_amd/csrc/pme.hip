@@ -46,7 +46,7 @@ struct md_pme_args {
 // one replica's table entry
 struct md_pme_table {
   md_pme_args P;
-  int n, np;
+  int n, np, nslots;
   const int64_t *pos;
   const float4 *param;
   const int *slot_atom;  // the nonbonded list's spatial order, 64-slot super-clusters
@@ -129,8 +129,10 @@ __device__ __forceinline__ int md_wrap_near(int d, int n) {
 }
 
 __global__ void __launch_bounds__(MD_SPREAD_THREADS) k_pme_spread(const md_pme_table *__restrict__ T, md_bmap map) {
+  // super-clusters laid out XCD by XCD: neighbouring ones add into the same
+  // mesh region from one L2
   int S;
-  const md_pme_table &E = md_pme_entry(T, map, S);
+  const md_pme_table &E = T[md_locate(map, md_xcd_block(blockIdx.x, gridDim.x), S)];
   const md_pme_args &P = E.P;
   int64_t *grid = E.grid;
   __shared__ unsigned long long tile[MD_SPREAD_TILE];
@@ -452,16 +454,20 @@ __global__ void __launch_bounds__(256) k_pme_z_inverse(const md_pme_table *__res
   }
 }
 
-// interpolate dE/dQ back onto the atoms: one thread per atom
+// interpolate dE/dQ back onto the atoms: one thread per slot of the
+// nonbonded list's spatial order, blocks laid out XCD by XCD, so the atoms
+// of one XCD's blocks read one region of the mesh through one L2
 __global__ void k_pme_gather(const md_pme_table *__restrict__ T, md_bmap map) {
   int blk;
-  const md_pme_table &E = md_pme_entry(T, map, blk);
+  const md_pme_table &E = T[md_locate(map, md_xcd_block(blockIdx.x, gridDim.x), blk)];
   const md_pme_args &P = E.P;
   const float *phi = E.phi;
   int64_t *force = E.force;
   const int np = E.np;
-  int i = blk * blockDim.x + threadIdx.x;
-  if (i >= E.n) return;
+  const int s = blk * blockDim.x + threadIdx.x;
+  if (s >= E.nslots) return;
+  const int i = E.slot_atom[s];
+  if (i < 0) return;
   float q = E.param[i].w;
   if (q == 0.0f) return;
   int base[3];
@@ -629,6 +635,7 @@ static md_pme_table pme_table(md_system *s) {
   E.P = pme_args(s);
   E.n = s->n;
   E.np = s->np;
+  E.nslots = s->nb.nslots;
   E.pos = s->pos;
   E.param = s->atom_param;
   E.slot_atom = s->slot_atom;
@@ -660,7 +667,7 @@ void md_pme_group_setup(md_group *g) {
     zrows.push_back(on ? (nx * ny + PME_BATCH - 1) / PME_BATCH : 0);
     ylines.push_back(on ? nx * kzb : 0);
     xlines.push_back(on ? ny * kzb : 0);
-    gather.push_back(on ? (s->n + 255) / 256 : 0);
+    gather.push_back(on ? (s->nb.nslots + 255) / 256 : 0);
     if (on) {
       mx = std::max(mx, nx);
       my = std::max(my, ny);

@@ -48,6 +48,8 @@ async function main() {
     expectError(() => native.createSystem(odd, { time_step: 0.002 }), 'NotSmooth');
     expectError(() => native.createSystem(Object.assign({}, desc, { n_particles: 0 }), { time_step: 0.002 }),
       'InvalidArgument');
+    // the barostat binding checks its handle before touching the library
+    expectError(() => native.attachBarostat({}, { frequency: 25 }), 'TypeError');
     // no GPU here: a valid System fails loudly, with no fallback
     expectError(() => new HipEngine(desc, { positions: Float64Array.from(fixture.positions) }), 'NoDevice');
     console.log('node binding (cpu): ok');
@@ -69,6 +71,19 @@ async function main() {
   expectError(() => engine.setBox([1.5, 1.5, 1.5]), 'CutoffTooLargeForBox');
   await engine.run(5);
   assert.strictEqual(engine.steps, s0 + 30);
+  // the Monte Carlo barostat on the device: every particle its own molecule
+  const n = desc.n_particles;
+  const start = Int32Array.from({ length: n + 1 }, (_, i) => i);
+  const atoms = Int32Array.from({ length: n }, (_, i) => i);
+  native.attachBarostat(engine.handle, { pressure: 1, temperature: 300, frequency: 5, moleculeStart: start,
+    moleculeAtoms: atoms, seed: 3 });
+  await engine.run(10);
+  const st = native.barostatState(engine.handle);
+  assert.strictEqual(st.attempts, 2);
+  assert.strictEqual(st.stepsSinceAttempt, 0);
+  assert(st.draws >= 2 && Number.isFinite(st.lastWork));
+  expectError(() => native.attachBarostat(engine.handle, { pressure: 1, temperature: 300, frequency: 5,
+    moleculeStart: Int32Array.from([0, 1]), moleculeAtoms: atoms, seed: 3 }), 'InvalidArgument');
   engine.destroy();
   console.log(`node binding (gpu): ok, direct ${e.groups[3]}, reciprocal ${e.groups[4]}, T ${sample.temperature}`);
 }

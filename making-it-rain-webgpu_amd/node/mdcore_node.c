@@ -370,6 +370,76 @@ static napi_value js_set_step(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
+/* attachBarostat(handle, {pressure, temperature, frequency, moleculeStart:
+ * Int32Array, moleculeAtoms: Int32Array, seed, volumeScale, draws,
+ * stepsSinceAttempt, windowAttempted, windowAccepted}): md_barostat_attach,
+ * so run() performs the Monte Carlo moves on the device (frequency 0
+ * detaches).  The molecules are upstream's findMolecules
+ * (src/engine/molecules.ts:71) as a CSR table. */
+static napi_value js_attach_barostat(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  md_system *s;
+  if (get_args(env, info, 2, argv) || get_handle(env, argv[0], &s)) return NULL;
+  napi_value obj = argv[1];
+  md_barostat_desc d;
+  memset(&d, 0, sizeof d);
+  d.pressure = num_field(env, obj, "pressure");
+  d.temperature = num_field(env, obj, "temperature");
+  d.frequency = (int32_t)num_field(env, obj, "frequency");
+  d.seed = (uint32_t)num_field(env, obj, "seed");
+  d.volume_scale = num_field(env, obj, "volumeScale");
+  d.draws = (int64_t)num_field(env, obj, "draws");
+  d.steps_since_attempt = (int32_t)num_field(env, obj, "stepsSinceAttempt");
+  d.window_attempted = (int32_t)num_field(env, obj, "windowAttempted");
+  d.window_accepted = (int32_t)num_field(env, obj, "windowAccepted");
+  if (d.frequency > 0) {
+    void *start = NULL, *atoms = NULL;
+    size_t nstart = 0, natoms = 0;
+    napi_value v;
+    napi_get_named_property(env, obj, "moleculeStart", &v);
+    if (typed(env, v, napi_int32_array, &start, &nstart)) return NULL;
+    napi_get_named_property(env, obj, "moleculeAtoms", &v);
+    if (typed(env, v, napi_int32_array, &atoms, &natoms)) return NULL;
+    if (nstart < 2) {
+      throw_type(env, "moleculeStart needs at least two entries");
+      return NULL;
+    }
+    d.n_molecules = (int32_t)(nstart - 1);
+    d.molecule_start = (const int32_t *)start;
+    d.molecule_atoms = (const int32_t *)atoms;
+  }
+  check(env, md_barostat_attach(s, &d));
+  return NULL;
+}
+
+/* barostatState(handle) -> md_barostat_state's fields */
+static napi_value js_barostat_state(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  md_system *s;
+  if (get_args(env, info, 1, argv) || get_handle(env, argv[0], &s)) return NULL;
+  md_barostat_state st;
+  if (check(env, md_barostat_get(s, &st))) return NULL;
+  napi_value out, v;
+  napi_create_object(env, &out);
+#define PUT_NUM(key, value)                   \
+  napi_create_double(env, (double)(value), &v); \
+  napi_set_named_property(env, out, key, v)
+  PUT_NUM("attempts", st.attempts);
+  PUT_NUM("accepted", st.accepted);
+  PUT_NUM("draws", st.draws);
+  PUT_NUM("stepsSinceAttempt", st.steps_since_attempt);
+  PUT_NUM("windowAttempted", st.window_attempted);
+  PUT_NUM("windowAccepted", st.window_accepted);
+  PUT_NUM("volumeScale", st.volume_scale);
+  PUT_NUM("lastAccepted", st.last_accepted);
+  PUT_NUM("lastEnergyBefore", st.last_energy_before);
+  PUT_NUM("lastEnergyAfter", st.last_energy_after);
+  PUT_NUM("lastWork", st.last_work);
+  PUT_NUM("lastLengthScale", st.last_length_scale);
+#undef PUT_NUM
+  return out;
+}
+
 /* run(handle, steps) -> Promise<void>, md_run on the thread pool */
 typedef struct {
   napi_async_work work;
@@ -464,6 +534,8 @@ static napi_value init(napi_env env, napi_value exports) {
   EXPORT("step", js_step);
   EXPORT("setStep", js_set_step);
   EXPORT("run", js_run);
+  EXPORT("attachBarostat", js_attach_barostat);
+  EXPORT("barostatState", js_barostat_state);
   return exports;
 }
 
