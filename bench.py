@@ -264,6 +264,9 @@ def run_ensemble(args):
     batch = HipBatch(engines)
     # the pair kernel's schedule for this batch, picked by timing (changes no bits)
     tune = batch.autotune(args.tune_steps) if args.tune_steps > 0 else None
+    if args.pair_schedule != "tuned":
+        batch._native.set_pair_schedule(compact=args.pair_schedule == "compact")
+        log(f"pair schedule set to {args.pair_schedule}")
     if tune:
         log(f"autotuned pair schedule: {tune['segment_entries']} entries per unit, {tune['pair_waves']} waves "
             f"({tune['ms_per_step']:.3f} ms/step phase by phase) vs the default {tune['default_segment_entries']}, "
@@ -576,6 +579,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--tune-steps", type=int, default=5,
                     help="steps timed per pair-kernel schedule by the autotuner (0: no tuning)")
+    ap.add_argument("--pair-schedule", choices=("tuned", "direct", "compact"), default="tuned",
+                    help="the pair kernel's evaluation: as the autotuner picked, or forced")
     ap.add_argument("--verlet-skin", type=float, default=None,
                     help="list skin in nm (default: the engine's, 0.15)")
     ap.add_argument("--max-batch", type=int, default=0,

@@ -242,20 +242,27 @@ md_err md_batch_profile_steps(md_batch *batch, int32_t steps, double *ms);
 
 /* The pair kernel's autotuner.  No counterpart upstream: the WebGPU loop
  * has fixed workgroup shapes (src/gpu/loop.ts:171).  It times the pair
- * kernel's schedules -- list entries per work unit (8, 16, 32) and resident
- * waves (2-4 per SIMD) -- on the group's own replicas, `steps` steps each,
- * keeps the fastest, and caches it per (replica count, atom-count bucket,
- * triclinic or not) for launch groups created later in the process.  The
- * schedule changes no bits; the timed steps are ordinary steps. */
+ * kernel's schedules -- list entries per work unit (8, 16, 32), resident
+ * waves (2-4 per SIMD), and the direct or the compacted evaluation (the
+ * in-cutoff pairs of two entries packed densely into full waves) -- on the
+ * group's own replicas, `steps` steps each, keeps the fastest, and caches it
+ * per (replica count, atom-count bucket, triclinic or not) for launch groups
+ * created later in the process.  No schedule changes any bit; the timed
+ * steps are ordinary steps. */
 typedef struct {
   int32_t segment_entries, pair_waves;                 /* the schedule picked */
   double ms_per_step;                                  /* its mean step, phase by phase */
   int32_t default_segment_entries, default_pair_waves; /* the schedule before tuning */
   double default_ms_per_step;
   int32_t candidates;                                  /* schedules timed */
+  int32_t compact_pairs, default_compact_pairs;        /* 1: the compacted evaluation */
 } md_schedule;
 md_err md_autotune(md_system *sys, int32_t steps, md_schedule *best);
 md_err md_batch_autotune(md_batch *batch, int32_t steps, md_schedule *best);
+/* Set a schedule directly (segment_entries, pair_waves -- 0 keeps the
+ * current -- and compact_pairs of `schedule`; the other fields are ignored). */
+md_err md_set_pair_schedule(md_system *sys, const md_schedule *schedule);
+md_err md_batch_set_pair_schedule(md_batch *batch, const md_schedule *schedule);
 
 /* Learned SHAKE warm start.  No counterpart upstream: it attaches where
  * `shake_cluster` starts (src/gpu/kernels/shake.wgsl:130-146), before the

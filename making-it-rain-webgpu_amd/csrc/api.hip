@@ -129,6 +129,7 @@ static void group_release(md_group *g) {
 // whose key has an entry starts from it.
 struct md_tune_entry {
   int seg_len, waves;
+  bool compact;
 };
 static std::map<std::tuple<int, int, int>, md_tune_entry> g_tuned;
 
@@ -145,7 +146,10 @@ static std::tuple<int, int, int> tune_key(const md_group *g) {
 
 static void apply_tuned_schedule(md_group *g) {
   auto it = g_tuned.find(tune_key(g));
-  if (it != g_tuned.end()) md_nb_set_schedule(g, it->second.seg_len, it->second.waves);
+  if (it != g_tuned.end()) {
+    md_nb_set_schedule(g, it->second.seg_len, it->second.waves);
+    md_nb_set_compact(g, it->second.compact);
+  }
 }
 
 // a launch group over `reps` (all on one device), on `stream` or its own
@@ -1074,19 +1078,27 @@ static void autotune_group(md_group *g, int steps, md_schedule *out) {
   if (!g->nb) throw md_error(MD_ERR_INVALID_ARGUMENT, "a system without a NonbondedForce has no pair kernel to tune");
   int seg0 = 0, waves0 = 0;
   md_nb_get_schedule(g, &seg0, &waves0);
+  const bool compact0 = md_nb_get_compact(g);
   int cus = 1;
   MD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
-  std::vector<std::pair<int, int>> candidates = {{seg0, waves0}};
-  for (int seg : {8, 16, 32})
-    for (int per_simd : {2, 3, 4}) {
-      std::pair<int, int> c{seg, cus * 4 * per_simd};
-      if (std::find(candidates.begin(), candidates.end(), c) == candidates.end()) candidates.push_back(c);
-    }
+  struct cand {
+    int seg, waves;
+    bool compact;
+    bool operator==(const cand &o) const { return seg == o.seg && waves == o.waves && compact == o.compact; }
+  };
+  std::vector<cand> candidates = {{seg0, waves0, compact0}};
+  for (int compact : {0, 1})
+    for (int seg : {8, 16, 32})
+      for (int per_simd : {2, 3, 4}) {
+        cand c{seg, cus * 4 * per_simd, compact != 0};
+        if (std::find(candidates.begin(), candidates.end(), c) == candidates.end()) candidates.push_back(c);
+      }
   double ms[MD_PHASE_COUNT];
   double best = 0.0, first = 0.0;
-  std::pair<int, int> pick = candidates[0];
+  cand pick = candidates[0];
   for (size_t k = 0; k < candidates.size(); ++k) {
-    md_nb_set_schedule(g, candidates[k].first, candidates[k].second);
+    md_nb_set_schedule(g, candidates[k].seg, candidates[k].waves);
+    md_nb_set_compact(g, candidates[k].compact);
     g->graph_stale = true;  // the pair grid is baked into a captured graph
     for (md_system *s : g->reps) s->need_rebuild = true;
     profile_group(g, 2, ms);  // the rebuild with the new unit length, and one more step
@@ -1099,19 +1111,32 @@ static void autotune_group(md_group *g, int steps, md_schedule *out) {
       pick = candidates[k];
     }
   }
-  md_nb_set_schedule(g, pick.first, pick.second);
+  md_nb_set_schedule(g, pick.seg, pick.waves);
+  md_nb_set_compact(g, pick.compact);
   g->graph_stale = true;
   for (md_system *s : g->reps) s->need_rebuild = true;
-  g_tuned[tune_key(g)] = md_tune_entry{pick.first, pick.second};
+  g_tuned[tune_key(g)] = md_tune_entry{pick.seg, pick.waves, pick.compact};
   if (out) {
-    out->segment_entries = pick.first;
-    out->pair_waves = pick.second;
+    out->segment_entries = pick.seg;
+    out->pair_waves = pick.waves;
     out->ms_per_step = best;
     out->default_segment_entries = seg0;
     out->default_pair_waves = waves0;
     out->default_ms_per_step = first;
     out->candidates = (int32_t)candidates.size();
+    out->compact_pairs = pick.compact ? 1 : 0;
+    out->default_compact_pairs = compact0 ? 1 : 0;
   }
+}
+
+static void set_pair_schedule(md_group *g, const md_schedule *sc) {
+  if (!g->nb) throw md_error(MD_ERR_INVALID_ARGUMENT, "a system without a NonbondedForce has no pair kernel");
+  int seg = 0, waves = 0;
+  md_nb_get_schedule(g, &seg, &waves);
+  md_nb_set_schedule(g, sc->segment_entries > 0 ? sc->segment_entries : seg, sc->pair_waves > 0 ? sc->pair_waves : waves);
+  md_nb_set_compact(g, sc->compact_pairs != 0);
+  g->graph_stale = true;
+  for (md_system *s : g->reps) s->need_rebuild = true;
 }
 
 struct md_batch {
@@ -1443,6 +1468,23 @@ md_err md_batch_autotune(md_batch *b, int32_t steps, md_schedule *best) {
   if (!b || steps <= 0) throw md_error(MD_ERR_INVALID_ARGUMENT, "bad arguments to md_batch_autotune");
   if (!b->group) throw md_error(MD_ERR_INVALID_ARGUMENT, "a system of this batch has been destroyed");
   autotune_group(b->group, steps, best);
+  MD_API_END
+}
+
+md_err md_set_pair_schedule(md_system *s, const md_schedule *schedule) {
+  MD_API_BEGIN
+  if (!s || !schedule) throw md_error(MD_ERR_INVALID_ARGUMENT, "null argument");
+  MD_HIP(hipSetDevice(s->device));
+  set_pair_schedule(s->self, schedule);
+  MD_API_END
+}
+
+md_err md_batch_set_pair_schedule(md_batch *b, const md_schedule *schedule) {
+  MD_API_BEGIN
+  if (!b || !schedule) throw md_error(MD_ERR_INVALID_ARGUMENT, "null argument");
+  if (!b->group) throw md_error(MD_ERR_INVALID_ARGUMENT, "a system of this batch has been destroyed");
+  MD_HIP(hipSetDevice(b->group->device));
+  set_pair_schedule(b->group, schedule);
   MD_API_END
 }
 

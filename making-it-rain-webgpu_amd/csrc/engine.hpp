@@ -323,6 +323,8 @@ void md_int_group_free(md_group *g);
 // ---- group launchers -----------------------------------------------------------
 void md_nb_set_schedule(md_group *g, int seg_len, int waves);  // pair-kernel schedule (autotuner)
 void md_nb_get_schedule(md_group *g, int *seg_len, int *waves);
+void md_nb_set_compact(md_group *g, bool compact);  // forces through the compacted pair schedule
+bool md_nb_get_compact(md_group *g);
 void md_nb_build(md_group *g);                  // rebuild kernels, gated per replica
 void md_nb_pairs(md_group *g, bool energy);
 void md_bonded(md_group *g, bool energy);       // bonded.hip

@@ -1144,6 +1144,230 @@ __device__ __forceinline__ void md_pair_segment(const md_pair_args &A, int seg, 
   md_atomic_add_i64(&A.slot_force[2 * A.nslots + is], az);
 }
 
+// ---- the compacted schedule ------------------------------------------------
+//
+// In an 8x8 block only about one lane in six holds a pair inside the cutoff,
+// yet the pair arithmetic costs the whole wave.  The compacted schedule
+// splits each entry's work in three:
+//   A. per active block, the distance, the in-cutoff ballot and, for each
+//      counted pair, its code (entry of the batch, block, lane) appended to a
+//      queue in LDS at a place given by the ballot's prefix count;
+//   C. the queue evaluated densely, 64 pairs per pass, each lane fetching its
+//      pair's i and j data from LDS tables, the force factor written back to
+//      LDS at the pair's own (entry, block, lane) place;
+//   D. per active block, each lane reading its pair's factor back (zero for a
+//      pair not counted) and accumulating exactly as the direct schedule does.
+// Two consecutive entries (always in the same unit: units are 8 entries
+// counted from the super-cluster's first) share one queue, so the dense
+// passes run fuller.  The factors are the same floating-point expression of
+// the same inputs, and D's sums run in the direct schedule's order, so both
+// schedules give the same bits.
+
+#define MD_COMPACT_BATCH 2
+
+struct md_pair_lds {
+  int4 ixyz[MD_SUPER];  // S's slots: frame x, y, z and the scaled charge's bits
+  float2 ise[MD_SUPER]; // scaled sigma/2 and 2 sqrt(eps)
+  int4 jxyz[MD_COMPACT_BATCH][MD_CLUSTER];  // the batch's j atoms in S's frame, scaled charge bits
+  float2 jse[MD_COMPACT_BATCH][MD_CLUSTER];
+  unsigned short queue[MD_COMPACT_BATCH * 512];  // (entry << 9) | (block << 6) | lane
+  float fs[MD_COMPACT_BATCH][512];               // the pair's force factor at (entry, block, lane)
+};
+
+__device__ __forceinline__ void md_pair_segment_compact(const md_pair_args &A, int seg, int lane, int il, int jl,
+                                                        md_pair_lds &L) {
+  const int4 sg = md_uniform_xyz(A.segments + seg);
+  const int S = __builtin_amdgcn_readfirstlane(sg.x), eb = __builtin_amdgcn_readfirstlane(sg.y),
+            ee = __builtin_amdgcn_readfirstlane(sg.z);
+  int xi[8], yi[8], zi[8];
+  float fx[8], fy[8], fz[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int4 p = A.slot_xyzq[S * MD_SUPER + q * MD_CLUSTER + il];
+    xi[q] = p.x;
+    yi[q] = p.y;
+    zi[q] = p.z;
+    fx[q] = fy[q] = fz[q] = 0.0f;
+  }
+  {
+    // the i table: lane l holds slot S*64 + l (block l/8, atom l%8)
+    const int4 p = A.slot_xyzq[S * MD_SUPER + lane];
+    const float2 lj = A.slot_lj[S * MD_SUPER + lane];
+    L.ixyz[lane] = make_int4(p.x, p.y, p.z, __float_as_int(__int_as_float(p.w) * MD_PAIR_Q_SCALE));
+    L.ise[lane] = make_float2(lj.x * MD_PAIR_S_SCALE, lj.y * MD_PAIR_E_SCALE);
+  }
+  int64_t ax = 0, ay = 0, az = 0;
+  int unit_end = min(eb + MD_SEGMENT, ee);
+  int4 ilo = make_int4(0, 0, 0, 0), ihi = ilo;
+  if (lane < 8) {
+    ilo = A.cluster_lo[S * 8 + lane];
+    ihi = A.cluster_hi[S * 8 + lane];
+  }
+  for (int e0 = eb; e0 < ee; e0 += MD_COMPACT_BATCH) {
+    const int nb = min(MD_COMPACT_BATCH, ee - e0);
+    int active[MD_COMPACT_BATCH];
+    uint64_t m[MD_COMPACT_BATCH][8];
+    unsigned xj[MD_COMPACT_BATCH], yj[MD_COMPACT_BATCH], zj[MD_COMPACT_BATCH];
+    int base = 0;
+    // A: distances, ballots and the queue
+#pragma unroll
+    for (int b = 0; b < MD_COMPACT_BATCH; ++b) {
+      active[b] = 0;
+      xj[b] = yj[b] = zj[b] = 0u;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) m[b][q] = 0;
+      if (b >= nb) continue;
+      const int e = e0 + b;
+      const int J = md_uniform(A.entry_j + e);
+      const int meta = md_uniform(A.entry_imask + e);
+      const int4 sh = md_uniform_xyz(A.entry_shift + e);
+      {
+        const int4 jlo = md_uniform_xyz(A.cluster_lo + J), jhi = md_uniform_xyz(A.cluster_hi + J);
+        const float dx = md_box_gap(ilo.x, ihi.x, (int)((unsigned)jlo.x + (unsigned)sh.x),
+                                    (int)((unsigned)jhi.x + (unsigned)sh.x));
+        const float dy = md_box_gap(ilo.y, ihi.y, (int)((unsigned)jlo.y + (unsigned)sh.y),
+                                    (int)((unsigned)jhi.y + (unsigned)sh.y));
+        const float dz = md_box_gap(ilo.z, ihi.z, (int)((unsigned)jlo.z + (unsigned)sh.z),
+                                    (int)((unsigned)jhi.z + (unsigned)sh.z));
+        const bool near = lane < 8 && fmaf(dx, dx, fmaf(dy, dy, dz * dz)) < A.prune2;
+        active[b] = meta & (int)(__ballot(near) & 0xffu);
+      }
+      if (!active[b]) continue;
+      const int4 pj = A.slot_xyzq[J * MD_CLUSTER + jl];
+      xj[b] = (unsigned)pj.x + (unsigned)sh.x;
+      yj[b] = (unsigned)pj.y + (unsigned)sh.y;
+      zj[b] = (unsigned)pj.z + (unsigned)sh.z;
+      if (lane < MD_CLUSTER) {  // lanes 0-7 hold j atoms 0-7
+        const float2 lj = A.slot_lj[J * MD_CLUSTER + jl];
+        L.jxyz[b][jl] = make_int4((int)xj[b], (int)yj[b], (int)zj[b],
+                                  __float_as_int(__int_as_float(pj.w) * MD_PAIR_Q_SCALE));
+        L.jse[b][jl] = make_float2(lj.x * MD_PAIR_S_SCALE, lj.y * MD_PAIR_E_SCALE);
+      }
+      const uint64_t *ex = A.entry_excl + (size_t)e * 8;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        if ((active[b] >> q) & 1) {
+          const uint64_t allowed = (meta & MD_META_CLEAN) ? ~0ull : md_uniform(ex + q);
+          const float dx = (float)(int)(xj[b] - (unsigned)xi[q]), dy = (float)(int)(yj[b] - (unsigned)yi[q]),
+                      dz = (float)(int)(zj[b] - (unsigned)zi[q]);
+          const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+          const uint64_t inc = allowed & __ballot(r2 < A.cutoff2);
+          m[b][q] = inc;
+          if (inc) {
+            if ((inc >> lane) & 1ull) {
+              const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(inc >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)inc, 0u));
+              L.queue[base + below] = (unsigned short)((b << 9) | (q << 6) | lane);
+            }
+            base += __popcll(inc);
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // C: the counted pairs, 64 per pass
+    for (int c0 = 0; c0 < base; c0 += 64) {
+      const int k = c0 + lane;
+      if (k < base) {
+        const int code = L.queue[k];
+        const int b = code >> 9, q = (code >> 6) & 7, pl = code & 63;
+        const int4 I = L.ixyz[q * MD_CLUSTER + (pl >> 3)];
+        const float2 Ise = L.ise[q * MD_CLUSTER + (pl >> 3)];
+        const int4 Jd = L.jxyz[b][pl & 7];
+        const float2 Jse = L.jse[b][pl & 7];
+        const float dx = (float)(int)((unsigned)Jd.x - (unsigned)I.x), dy = (float)(int)((unsigned)Jd.y - (unsigned)I.y),
+                    dz = (float)(int)((unsigned)Jd.z - (unsigned)I.z);
+        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        float en = 0.0f;
+        int cnt = 0;
+        L.fs[b][q * 64 + pl] = md_pair<false>(r2, __int_as_float(I.w), Ise.x, Ise.y, __int_as_float(Jd.w), Jse.x,
+                                              Jse.y, true, A, en, cnt);
+      }
+    }
+    __syncthreads();
+    // D: back in the block layout, summed in the direct schedule's order
+#pragma unroll
+    for (int b = 0; b < MD_COMPACT_BATCH; ++b) {
+      if (b >= nb) continue;
+      const int e = e0 + b;
+      float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+      if (active[b]) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const uint64_t inc = m[b][q];
+          if (inc == 0) continue;
+          const float f = ((inc >> lane) & 1ull) ? L.fs[b][q * 64 + lane] : 0.0f;
+          const float dx = (float)(int)(xj[b] - (unsigned)xi[q]), dy = (float)(int)(yj[b] - (unsigned)yi[q]),
+                      dz = (float)(int)(zj[b] - (unsigned)zi[q]);
+          fx[q] = fmaf(-f, dx, fx[q]);
+          fy[q] = fmaf(-f, dy, fy[q]);
+          fz[q] = fmaf(-f, dz, fz[q]);
+          gx = fmaf(f, dx, gx);
+          gy = fmaf(f, dy, gy);
+          gz = fmaf(f, dz, gz);
+        }
+        gx = md_sum_same_j(gx);
+        gy = md_sum_same_j(gy);
+        gz = md_sum_same_j(gz);
+      }
+      if (lane < 8) {
+        float *r = A.j_force + (size_t)e * 24 + lane * 3;
+        r[0] = gx;
+        r[1] = gy;
+        r[2] = gz;
+      }
+      if (e + 1 == unit_end) {
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          float x, y, z;
+          if (q < 4) {
+            x = md_sum_same_i<false>(fx[q]);
+            y = md_sum_same_i<false>(fy[q]);
+            z = md_sum_same_i<false>(fz[q]);
+          } else {
+            x = md_sum_same_i<true>(fx[q]);
+            y = md_sum_same_i<true>(fy[q]);
+            z = md_sum_same_i<true>(fz[q]);
+          }
+          if (jl == q) {
+            sx = x;
+            sy = y;
+            sz = z;
+          }
+          fx[q] = fy[q] = fz[q] = 0.0f;
+        }
+        ax += md_to_fixed(sx);
+        ay += md_to_fixed(sy);
+        az += md_to_fixed(sz);
+        unit_end = min(unit_end + MD_SEGMENT, ee);
+      }
+    }
+    __syncthreads();  // the next batch rewrites the tables this one read
+  }
+  const int is = (S * 8 + jl) * MD_CLUSTER + il;
+  md_atomic_add_i64(&A.slot_force[is], ax);
+  md_atomic_add_i64(&A.slot_force[A.nslots + is], ay);
+  md_atomic_add_i64(&A.slot_force[2 * A.nslots + is], az);
+}
+
+// The compacted schedule's kernel (forces only; energies use the direct one).
+template <bool BATCH>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_nb_pairs_compact(const md_nb_args *__restrict__ T, md_bmap seg_map, int *cursor) {
+  __shared__ md_pair_lds L;
+  const int nseg = seg_map.start[seg_map.R];
+  const int lane = threadIdx.x, il = lane >> 3, jl = lane & 7;
+  for (;;) {
+    // as in k_nb_pairs: every lane in the atomic, the index made uniform
+    const int g = __builtin_amdgcn_readfirstlane(atomicAdd(cursor, lane == 0 ? 1 : 0));
+    if (g >= nseg) break;
+    int seg;
+    const md_pair_args &A = T[md_locate(seg_map, g, seg)].P;
+    md_pair_segment_compact(A, seg, lane, il, jl, L);
+    __syncthreads();  // the next segment rewrites the i table
+  }
+}
+
 // BATCH only names the instance: a launch over several replicas runs
 // k_nb_pairs<·, true>, one replica <·, false>, the same code, so a kernel
 // trace reports the batched launches on their own.
@@ -1530,6 +1754,7 @@ struct md_nb_group {
   int *cursor = nullptr;
   int pair_grid[2] = {1, 1};
   int seg_len = MD_SEGMENT;
+  bool compact = false;  // forces through the compacted schedule
 };
 
 static md_nb_group &nbg(md_group *g) { return *reinterpret_cast<md_nb_group *>(g->nb); }
@@ -1598,6 +1823,9 @@ void md_nb_get_schedule(md_group *g, int *seg_len, int *waves) {
   *waves = nbg(g).pair_grid[0];
 }
 
+void md_nb_set_compact(md_group *g, bool compact) { nbg(g).compact = compact; }
+bool md_nb_get_compact(md_group *g) { return nbg(g).compact; }
+
 void md_nb_group_refresh(md_group *g) {
   std::vector<md_nb_args> host;
   for (md_system *s : g->reps) host.push_back(nb_args(s, nbg(g).seg_len));
@@ -1656,6 +1884,11 @@ void md_nb_pairs(md_group *g, bool energy) {
       k_nb_pairs<true, true><<<G.pair_grid[1], 64, 0, st>>>(G.table, G.seg.map, G.cursor);
     else
       k_nb_pairs<true, false><<<G.pair_grid[1], 64, 0, st>>>(G.table, G.seg.map, G.cursor);
+  } else if (G.compact) {
+    if (batch)
+      k_nb_pairs_compact<true><<<G.pair_grid[0], 64, 0, st>>>(G.table, G.seg.map, G.cursor);
+    else
+      k_nb_pairs_compact<false><<<G.pair_grid[0], 64, 0, st>>>(G.table, G.seg.map, G.cursor);
   } else {
     if (batch)
       k_nb_pairs<false, true><<<G.pair_grid[0], 64, 0, st>>>(G.table, G.seg.map, G.cursor);

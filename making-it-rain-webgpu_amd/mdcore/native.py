@@ -109,7 +109,8 @@ class RunOptionsC(C.Structure):
 class Schedule(C.Structure):
     _fields_ = [("segment_entries", C.c_int32), ("pair_waves", C.c_int32), ("ms_per_step", C.c_double),
                 ("default_segment_entries", C.c_int32), ("default_pair_waves", C.c_int32),
-                ("default_ms_per_step", C.c_double), ("candidates", C.c_int32)]
+                ("default_ms_per_step", C.c_double), ("candidates", C.c_int32), ("compact_pairs", C.c_int32),
+                ("default_compact_pairs", C.c_int32)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -185,6 +186,8 @@ def library():
         L.md_autotune.argtypes = [C.c_void_p, C.c_int32, C.POINTER(Schedule)]
         L.md_minimise.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.POINTER(MinimiseResultC)]
         L.md_batch_autotune.argtypes = [C.c_void_p, C.c_int32, C.POINTER(Schedule)]
+        L.md_set_pair_schedule.argtypes = [C.c_void_p, C.POINTER(Schedule)]
+        L.md_batch_set_pair_schedule.argtypes = [C.c_void_p, C.POINTER(Schedule)]
         L.md_barostat_attach.argtypes = [C.c_void_p, C.POINTER(BarostatDescC)]
         L.md_barostat_get.argtypes = [C.c_void_p, C.POINTER(BarostatStateC)]
         _fp = P(C.c_float)
@@ -404,6 +407,11 @@ class NativeSystem:
         check(library().md_autotune(self.handle, int(steps), C.byref(out)))
         return out.as_dict()
 
+    def set_pair_schedule(self, segment_entries: int = 0, pair_waves: int = 0, compact: bool = False):
+        """md_set_pair_schedule (0 keeps the current length or wave count)."""
+        sc = Schedule(segment_entries=int(segment_entries), pair_waves=int(pair_waves), compact_pairs=int(bool(compact)))
+        check(library().md_set_pair_schedule(self.handle, C.byref(sc)))
+
     def minimise(self, max_iterations: int = 0, tolerance: float = 10.0) -> dict:
         """md_minimise: L-BFGS with constraint springs, searched on the device."""
         out = MinimiseResultC()
@@ -472,6 +480,11 @@ class NativeBatch:
         out = Schedule()
         check(library().md_batch_autotune(self._h, int(steps), C.byref(out)))
         return out.as_dict()
+
+    def set_pair_schedule(self, segment_entries: int = 0, pair_waves: int = 0, compact: bool = False):
+        """md_batch_set_pair_schedule (0 keeps the current length or wave count)."""
+        sc = Schedule(segment_entries=int(segment_entries), pair_waves=int(pair_waves), compact_pairs=int(bool(compact)))
+        check(library().md_batch_set_pair_schedule(self._h, C.byref(sc)))
 
 
 PHASE_COUNT = 12
