@@ -416,9 +416,16 @@ __device__ __forceinline__ bool md_boxes_within(float dx, float dy, float dz, fl
 // shift and of its negative have opposite signs: a super-cluster that spans
 // a periodic boundary would otherwise centre them elsewhere, and keeping
 // the positive codes would then keep both of some +/- pairs.
+//
+// Images whose f32 estimate lies at or beyond `skip` (nm) are not visited:
+// the estimate (the reduced difference plus the lattice steps, in f32) is
+// within 1e-5 nm of the exact image for boxes within 32 nm, so with `skip`
+// widened by 1e-3 nm past the caller's own exact test no image that test
+// would keep is skipped, and the int64 image arithmetic runs only for the
+// few images near enough to matter.
 template <typename F>
 __device__ void md_for_each_image(const md_build_ctx &c, int64_t dx, int64_t dy, int64_t dz, bool enumerate,
-                                  bool own, const int *K, F &&visit) {
+                                  bool own, const int *K, float skip, F &&visit) {
   if (!own) md_min_image_fixed(dx, dy, dz, c.box);
   if (!enumerate) {
     visit(dx, dy, dz, 0);
@@ -426,10 +433,16 @@ __device__ void md_for_each_image(const md_build_ctx &c, int64_t dx, int64_t dy,
   }
   const int na = 2 * K[0] + 1, nb = 2 * K[1] + 1;
   const int centre = (K[2] * nb + K[1]) * na + K[0];
+  const float fx0 = md_from_fixed(dx), fy0 = md_from_fixed(dy), fz0 = md_from_fixed(dz);
+  const float skip2 = skip * skip;
   int code = 0;
   for (int kc = -K[2]; kc <= K[2]; ++kc)
     for (int kb = -K[1]; kb <= K[1]; ++kb)
       for (int ka = -K[0]; ka <= K[0]; ++ka, ++code) {
+        const float gx = fmaf((float)kc, c.box.c[0], fmaf((float)kb, c.box.b[0], fmaf((float)ka, c.box.a[0], fx0)));
+        const float gy = fmaf((float)kc, c.box.c[1], fmaf((float)kb, c.box.b[1], fy0));
+        const float gz = fmaf((float)kc, c.box.c[2], fz0);
+        if (fmaf(gx, gx, fmaf(gy, gy, gz * gz)) >= skip2) continue;
         int64_t ex = dx + ka * c.box.fa[0] + kb * c.box.fb[0] + kc * c.box.fc[0];
         int64_t ey = dy + kb * c.box.fb[1] + kc * c.box.fc[1];
         int64_t ez = dz + kc * c.box.fc[2];
@@ -468,7 +481,7 @@ __device__ __forceinline__ void md_candidate(const md_build_ctx &c, const md_lis
   // clusters keep their unshifted frame (see md_for_each_image)
   int64_t dx = jx - V.ox, dy = jy - V.oy, dz = jz - V.oz;
   bool own = J < V.first + 8;
-  md_for_each_image(c, dx, dy, dz, enumerate, own, K, [&](int64_t ex, int64_t ey, int64_t ez, int code) {
+  md_for_each_image(c, dx, dy, dz, enumerate, own, K, reach + 1e-3f, [&](int64_t ex, int64_t ey, int64_t ez, int code) {
     // the lattice vector that moves J into this image
     int64_t shx = ex - dx, shy = ey - dy, shz = ez - dz;
     float ox = md_from_fixed(jx + shx - V.ox), oy = md_from_fixed(jy + shy - V.oy), oz = md_from_fixed(jz + shz - V.oz);
@@ -583,13 +596,18 @@ __global__ void __launch_bounds__(64) k_nb_list(const md_nb_args *__restrict__ T
           int64_t dx = L.super_origin[SJ] - ox, dy = L.super_origin[c.nsuper + SJ] - oy,
                   dz = L.super_origin[2 * c.nsuper + SJ] - oz;
           md_min_image_fixed(dx, dy, dz, c.box);
+          // f32 images of the reduced difference, with a 1e-3 nm margin over
+          // their rounding (a candidate kept in excess makes no entry: each
+          // of its clusters still meets the exact tests below)
+          const float fx0 = md_from_fixed(dx), fy0 = md_from_fixed(dy), fz0 = md_from_fixed(dz);
+          const float limm = lim + 1e-3f, lim2 = limm * limm;
           for (int kc = -Kp[2]; kc <= Kp[2]; ++kc)
             for (int kb = -Kp[1]; kb <= Kp[1]; ++kb)
               for (int ka = -Kp[0]; ka <= Kp[0]; ++ka) {
-                float fx = md_from_fixed(dx + ka * c.box.fa[0] + kb * c.box.fb[0] + kc * c.box.fc[0]);
-                float fy = md_from_fixed(dy + kb * c.box.fb[1] + kc * c.box.fc[1]);
-                float fz = md_from_fixed(dz + kc * c.box.fc[2]);
-                near |= fx * fx + fy * fy + fz * fz < lim * lim;
+                const float fx = fmaf((float)kc, c.box.c[0], fmaf((float)kb, c.box.b[0], fmaf((float)ka, c.box.a[0], fx0)));
+                const float fy = fmaf((float)kc, c.box.c[1], fmaf((float)kb, c.box.b[1], fy0));
+                const float fz = fmaf((float)kc, c.box.c[2], fz0);
+                near |= fmaf(fx, fx, fmaf(fy, fy, fz * fz)) < lim2;
               }
         }
       }
