@@ -1,4 +1,6 @@
-"""Force comparison that tolerates pairs at the cutoff.
+"""Force comparison that tolerates pairs at the cutoff (test infrastructure:
+the parity tests and the benchmark's after-run force check use it as the
+checker; the product path never does).
 
 The device decides r^2 < rc^2 in f32 on exact frame differences, the oracle
 in f64; a non-excluded pair within ~1e-7 nm of the cutoff can fall on either
