@@ -270,9 +270,11 @@ def run_ensemble(args):
     batch = HipBatch(engines)
     # the pair kernel's schedule for this batch, picked by timing (changes no bits)
     tune = batch.autotune(args.tune_steps) if args.tune_steps > 0 else None
-    if args.pair_schedule != "tuned":
-        batch._native.set_pair_schedule(compact=args.pair_schedule == "compact")
-        log(f"pair schedule set to {args.pair_schedule}")
+    if args.pair_schedule != "tuned" or args.pair_segment > 0:
+        batch._native.set_pair_schedule(segment_entries=args.pair_segment,
+                                        compact=args.pair_schedule == "compact" or
+                                        (args.pair_schedule == "tuned" and bool(tune and tune["compact_pairs"])))
+        log(f"pair schedule set to {args.pair_schedule}, {args.pair_segment or 'unchanged'} entries per unit")
     if tune:
         log(f"autotuned pair schedule: {tune['segment_entries']} entries per unit, {tune['pair_waves']} waves "
             f"({tune['ms_per_step']:.3f} ms/step phase by phase) vs the default {tune['default_segment_entries']}, "
@@ -494,6 +496,8 @@ def run_traffic_probe(args):
                          RunOptions(timeStep=0.0005, friction=10.0, temperature=r.temperature,
                                     constraintTolerance=1e-6, seed=r.seed, maxBatch=1)) for r in reps]
     batch = HipBatch(engines)
+    if args.pair_segment > 0:
+        batch._native.set_pair_schedule(segment_entries=args.pair_segment)
     batch.run(args.traffic_probe)
     device_synchronize()
     log(f"traffic probe: {args.traffic_probe} batched steps of {len(engines)} replicas")
@@ -592,6 +596,8 @@ def main():
                     help="steps timed per pair-kernel schedule by the autotuner (0: no tuning)")
     ap.add_argument("--pair-schedule", choices=("tuned", "direct", "compact"), default="tuned",
                     help="the pair kernel's evaluation: as the autotuner picked, or forced")
+    ap.add_argument("--pair-segment", type=int, default=0,
+                    help="list entries per pair-kernel work unit, forced (0: tuned, or the default in a probe)")
     ap.add_argument("--verlet-skin", type=float, default=None,
                     help="list skin in nm (default: the engine's, 0.15)")
     ap.add_argument("--max-batch", type=int, default=0,

@@ -242,7 +242,7 @@ md_err md_batch_profile_steps(md_batch *batch, int32_t steps, double *ms);
 
 /* The pair kernel's autotuner.  No counterpart upstream: the WebGPU loop
  * has fixed workgroup shapes (src/gpu/loop.ts:171).  It times the pair
- * kernel's schedules -- list entries per work unit (8, 16, 32), resident
+ * kernel's schedules -- list entries per work unit (8 to 128), resident
  * waves (2-4 per SIMD), and the direct or the compacted evaluation (the
  * in-cutoff pairs of two entries packed densely into full waves) -- on the
  * group's own replicas, `steps` steps each, keeps the fastest, and caches it

@@ -1087,12 +1087,15 @@ static void autotune_group(md_group *g, int steps, md_schedule *out) {
     bool operator==(const cand &o) const { return seg == o.seg && waves == o.waves && compact == o.compact; }
   };
   std::vector<cand> candidates = {{seg0, waves0, compact0}};
-  for (int compact : {0, 1})
-    for (int seg : {8, 16, 32})
-      for (int per_simd : {2, 3, 4}) {
-        cand c{seg, cus * 4 * per_simd, compact != 0};
-        if (std::find(candidates.begin(), candidates.end(), c) == candidates.end()) candidates.push_back(c);
-      }
+  auto add = [&](cand c) {
+    if (std::find(candidates.begin(), candidates.end(), c) == candidates.end()) candidates.push_back(c);
+  };
+  // the direct evaluation over unit lengths and resident waves (a longer
+  // unit means fewer i-force flushes, a shorter one finer load balance) ...
+  for (int seg : {8, 16, 32, 64, 128})
+    for (int per_simd : {2, 3, 4}) add(cand{seg, cus * 4 * per_simd, false});
+  // ... and the compacted one at the usual shape
+  add(cand{32, cus * 16, true});
   double ms[MD_PHASE_COUNT];
   double best = 0.0, first = 0.0;
   cand pick = candidates[0];

@@ -1829,8 +1829,8 @@ void md_nb_group_setup(md_group *g) {
 // from the counter.  waves <= 0 keeps the current grid.
 void md_nb_set_schedule(md_group *g, int seg_len, int waves) {
   auto &G = nbg(g);
-  if (seg_len % MD_SEGMENT || seg_len < MD_SEGMENT || seg_len > 8 * MD_SEGMENT)
-    throw md_error(MD_ERR_INVALID_ARGUMENT, "segment length must be 8, 16, 32 or 64 entries");
+  if (seg_len % MD_SEGMENT || seg_len < MD_SEGMENT || seg_len > 32 * MD_SEGMENT)
+    throw md_error(MD_ERR_INVALID_ARGUMENT, "segment length must be a multiple of 8 entries, at most 256");
   G.seg_len = seg_len;
   if (waves > 0) G.pair_grid[0] = waves;
   md_nb_group_refresh(g);

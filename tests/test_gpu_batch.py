@@ -151,7 +151,7 @@ def test_autotuner_picks_a_schedule_and_the_batch_goes_on(replicas):
     best = batch.autotune(5)
     print("autotuned schedule", best)
     assert best["candidates"] >= 9
-    assert best["segment_entries"] in (8, 16, 32)
+    assert best["segment_entries"] in (8, 16, 32, 64, 128)
     assert best["pair_waves"] > 0
     assert best["ms_per_step"] <= best["default_ms_per_step"]
     batch.run(60)
