@@ -53,7 +53,7 @@ PAIR_FLOPS = 76            # algorithmic FP32 flops per in-cutoff pair
 # WRITE_SIZE from two rocprofv3 --pmc passes over `bench.py --traffic-probe 3`,
 # each scaled by its reading for a 256 MiB device copy
 # (profiles/k_nb_pairs_ensemble64_traffic.md)
-PAIR_TRAFFIC_BYTES = 672.4e6
+PAIR_TRAFFIC_BYTES = 674.9e6
 DT_PS = 0.002
 METRIC = "aggregate ns/day at 2 fs (batched+learned vs plain, same GPU, fixed force error)"
 
