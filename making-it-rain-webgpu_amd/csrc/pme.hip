@@ -454,6 +454,79 @@ __global__ void __launch_bounds__(256) k_pme_z_inverse(const md_pme_table *__res
   }
 }
 
+// The z and y passes of one x-plane in one workgroup, the plane kept in LDS
+// between them (meshes whose plane fits: 2 ny nz complex values).  The same
+// line transforms as k_pme_z_forward then k_pme_y, with the same inputs, so
+// the same bits; one launch and one trip through memory fewer.
+__global__ void __launch_bounds__(256) k_pme_yz_forward(const md_pme_table *__restrict__ T, md_bmap map) {
+  extern __shared__ float2 lds[];
+  int x;
+  const md_pme_table &E = md_pme_entry(T, map, x);
+  const md_pme_args &P = E.P;
+  int64_t *grid = E.grid;
+  float2 *cgrid = E.cgrid;
+  const int ny = P.ny, nz = P.nz, nzc = P.nzc;
+  float2 *a = lds, *b = lds + ny * nz;
+  for (int idx = threadIdx.x; idx < ny * nz; idx += blockDim.x) {
+    const int y = idx / nz, z = idx - y * nz;
+    const size_t at = ((size_t)x * ny + y) * nz + z;
+    const int64_t v = grid[at];
+    grid[at] = 0;
+    a[y * nz + z] = make_float2((float)((double)v * 2.3283064365386963e-10), 0.0f);
+  }
+  __syncthreads();
+  float2 *res = md_lds_fft(a, b, ny, P.pz, P.tw, -1.0f);  // [y][kz]
+  float2 *other = res == a ? b : a;
+  for (int idx = threadIdx.x; idx < nzc * ny; idx += blockDim.x) {
+    const int kz = idx / ny, y = idx - kz * ny;
+    other[kz * ny + y] = res[y * nz + kz];
+  }
+  __syncthreads();
+  float2 *res2 = md_lds_fft(other, res, nzc, P.py, P.tw, -1.0f);  // [kz][ky]
+  for (int idx = threadIdx.x; idx < ny * nzc; idx += blockDim.x) {
+    const int ky = idx / nzc, kz = idx - ky * nzc;
+    cgrid[((size_t)x * ny + ky) * nzc + kz] = res2[kz * ny + ky];
+  }
+}
+
+// The inverse y and z passes of one x-plane (k_pme_y with +1, then
+// k_pme_z_inverse), the plane kept in LDS between them.
+__global__ void __launch_bounds__(256) k_pme_yz_inverse(const md_pme_table *__restrict__ T, md_bmap map) {
+  extern __shared__ float2 lds[];
+  int x;
+  const md_pme_table &E = md_pme_entry(T, map, x);
+  const md_pme_args &P = E.P;
+  const float2 *cgrid = E.cgrid;
+  float *phi = E.phi;
+  const int ny = P.ny, nz = P.nz, nzc = P.nzc;
+  float2 *a = lds, *b = lds + ny * nz;
+  for (int idx = threadIdx.x; idx < nzc * ny; idx += blockDim.x) {
+    const int kz = idx / ny, y = idx - kz * ny;
+    a[kz * ny + y] = cgrid[((size_t)x * ny + y) * nzc + kz];
+  }
+  __syncthreads();
+  float2 *res = md_lds_fft(a, b, nzc, P.py, P.tw, 1.0f);  // [kz][y]
+  float2 *other = res == a ? b : a;
+  for (int idx = threadIdx.x; idx < ny * nz; idx += blockDim.x) {
+    const int y = idx / nz, z = idx - y * nz;
+    float2 v;
+    if (z < nzc) {
+      v = res[z * ny + y];
+    } else {
+      // x and y are back in real space: g(x, y, -kz) = conj(g(x, y, kz))
+      const float2 w = res[(nz - z) * ny + y];
+      v = make_float2(w.x, -w.y);
+    }
+    other[y * nz + z] = v;
+  }
+  __syncthreads();
+  float2 *res2 = md_lds_fft(other, res, ny, P.pz, P.tw, 1.0f);  // [y][z]
+  for (int idx = threadIdx.x; idx < ny * nz; idx += blockDim.x) {
+    const int y = idx / nz, z = idx - y * nz;
+    phi[((size_t)x * ny + y) * nz + z] = res2[y * nz + z].x;
+  }
+}
+
 // interpolate dE/dQ back onto the atoms: one thread per slot of the
 // nonbonded list's spatial order, blocks laid out XCD by XCD, so the atoms
 // of one XCD's blocks read one region of the mesh through one L2
@@ -649,8 +722,10 @@ static md_pme_table pme_table(md_system *s) {
 
 struct md_pme_group {
   md_pme_table *table = nullptr;
-  md_launch spread, zrows, ylines, xlines, gather;
+  md_launch spread, zrows, ylines, xlines, gather, planes;
   size_t lz = 0, ly = 0, lx = 0;  // dynamic LDS of the three passes
+  size_t lyz = 0;                 // dynamic LDS of a whole x-plane
+  bool fused_yz = false;          // the y and z passes per x-plane in one launch
 };
 
 static md_pme_group &pg(md_group *g) { return *reinterpret_cast<md_pme_group *>(g->pme); }
@@ -658,7 +733,7 @@ static md_pme_group &pg(md_group *g) { return *reinterpret_cast<md_pme_group *>(
 void md_pme_group_setup(md_group *g) {
   auto *G = new md_pme_group();
   g->pme = G;
-  std::vector<int> spread, zrows, ylines, xlines, gather;
+  std::vector<int> spread, zrows, ylines, xlines, gather, planes;
   int mx = 0, my = 0, mz = 0;
   for (md_system *s : g->reps) {
     bool on = s->has_nonbonded;
@@ -668,6 +743,7 @@ void md_pme_group_setup(md_group *g) {
     ylines.push_back(on ? nx * kzb : 0);
     xlines.push_back(on ? ny * kzb : 0);
     gather.push_back(on ? (s->nb.nslots + 255) / 256 : 0);
+    planes.push_back(on ? nx : 0);
     if (on) {
       mx = std::max(mx, nx);
       my = std::max(my, ny);
@@ -679,6 +755,9 @@ void md_pme_group_setup(md_group *g) {
   G->ylines = md_make_launch(g, ylines);
   G->xlines = md_make_launch(g, xlines);
   G->gather = md_make_launch(g, gather);
+  G->planes = md_make_launch(g, planes);
+  G->lyz = sizeof(float2) * 2 * (size_t)my * mz;
+  G->fused_yz = G->lyz <= 64 * 1024;  // up to 64^3 meshes
   G->lz = sizeof(float2) * 2 * PME_BATCH * mz;
   G->ly = sizeof(float2) * 2 * PME_BATCH * my;
   G->lx = sizeof(float2) * 2 * PME_BATCH * mx;
@@ -726,14 +805,22 @@ void md_pme(md_group *g, bool energy) {
   const md_pme_table *T = G.table;
   k_pme_spread<<<G.spread.total, MD_SPREAD_THREADS, 0, st>>>(T, G.spread.map);
   md_mark(g, MD_PHASE_FFT);
-  k_pme_z_forward<<<G.zrows.total, 256, G.lz, st>>>(T, G.zrows.map);
-  k_pme_y<<<G.ylines.total, 256, G.ly, st>>>(T, G.ylines.map, -1.0f);
+  if (G.fused_yz) {
+    k_pme_yz_forward<<<G.planes.total, 256, G.lyz, st>>>(T, G.planes.map);
+  } else {
+    k_pme_z_forward<<<G.zrows.total, 256, G.lz, st>>>(T, G.zrows.map);
+    k_pme_y<<<G.ylines.total, 256, G.ly, st>>>(T, G.ylines.map, -1.0f);
+  }
   if (energy)
     k_pme_x_convolve<true><<<G.xlines.total, 256, G.lx, st>>>(T, G.xlines.map);
   else
     k_pme_x_convolve<false><<<G.xlines.total, 256, G.lx, st>>>(T, G.xlines.map);
-  k_pme_y<<<G.ylines.total, 256, G.ly, st>>>(T, G.ylines.map, 1.0f);
-  k_pme_z_inverse<<<G.zrows.total, 256, G.lz, st>>>(T, G.zrows.map);
+  if (G.fused_yz) {
+    k_pme_yz_inverse<<<G.planes.total, 256, G.lyz, st>>>(T, G.planes.map);
+  } else {
+    k_pme_y<<<G.ylines.total, 256, G.ly, st>>>(T, G.ylines.map, 1.0f);
+    k_pme_z_inverse<<<G.zrows.total, 256, G.lz, st>>>(T, G.zrows.map);
+  }
   md_mark(g, MD_PHASE_GATHER);
   k_pme_gather<<<G.gather.total, 256, 0, st>>>(T, G.gather.map);
   MD_HIP(hipGetLastError());
