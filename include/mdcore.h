@@ -379,6 +379,19 @@ md_err md_probe_fixed_point_noalloc(const float *values, int32_t n, int64_t *fix
  * numbering (src/gpu/cell-list.ts:136-155); `cells` holds the grid's size. */
 md_err md_probe_cell_order(const int32_t dims[3], int32_t *cells);
 
+/* The pair kernel's real-space Coulomb polynomials for an Ewald splitting
+ * parameter alpha (nm^-1) and a cutoff (nm), host only.  With u = alpha^2 r^2
+ * the kernel takes F/r per charge product as 1/r^3 + sum_k force[k] v^k and
+ * E as 1/r + sum_k energy[k] v^k, v = scale r^2 - 1 (13 terms each, f32);
+ * *ok says whether that f32 evaluation is within 1e-6 of 1/r^3 (force) and
+ * of 1/r (energy) over [0, cutoff], the errors are in *force_err and
+ * *energy_err; where it is not, the kernel evaluates erfc and exp.  Stands in
+ * for the screened Coulomb term of the reference's pair kernel
+ * (src/gpu/kernels/nonbonded-pairs.wgsl:203-212, erfc_approx in
+ * src/gpu/kernels/erfc.wgsl; its bound in test/erf.test.ts:68-89). */
+md_err md_probe_ewald_fit(double alpha, double cutoff, float *scale, float *force, float *energy, int32_t *ok,
+                          double *force_err, double *energy_err);
+
 /* Fault injection for host-side failure tests: the next `after` kernel
  * launches succeed, then every call returns MD_ERR_FAULT_INJECTED until
  * re-armed with a negative value.  No kernel is altered. */

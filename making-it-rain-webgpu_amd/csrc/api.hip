@@ -1419,6 +1419,22 @@ md_err md_probe_cell_order(const int32_t dims[3], int32_t *cells) {
   MD_API_END
 }
 
+md_err md_probe_ewald_fit(double alpha, double cutoff, float *scale, float *force, float *energy, int32_t *ok,
+                          double *force_err, double *energy_err) {
+  MD_API_BEGIN
+  if (!scale || !force || !energy || !ok || !force_err || !energy_err || !(alpha > 0.0) || !(cutoff > 0.0) ||
+      !std::isfinite(alpha) || !std::isfinite(cutoff))
+    throw md_error(MD_ERR_INVALID_ARGUMENT, "md_probe_ewald_fit: alpha and cutoff must be positive");
+  const md_ewald_poly P = md_ewald_fit(alpha, cutoff);
+  *scale = P.scale;
+  std::copy(P.force, P.force + MD_EWALD_TERMS, force);
+  std::copy(P.energy, P.energy + MD_EWALD_TERMS, energy);
+  *ok = P.ok;
+  *force_err = P.force_err;
+  *energy_err = P.energy_err;
+  MD_API_END
+}
+
 md_err md_inject_fault(md_system *s, int32_t after) {
   MD_API_BEGIN
   if (!s) throw md_error(MD_ERR_INVALID_ARGUMENT, "null handle");

@@ -85,10 +85,27 @@ __host__ __device__ inline double md_energy_groups(const int64_t *e, const md_en
   return total;
 }
 
+// The pair kernel's real-space Coulomb polynomials (md_ewald_fit, in nm).
+// With u = alpha^2 r^2 the Ewald direct-space force and energy per charge
+// product are
+//   F/r = 1/r^3 + alpha^3 g(u),  g(u) = ((2/sqrt(pi)) e^-u - erf(sqrt u)/sqrt u)/u
+//   E   = 1/r   - alpha h(u),    h(u) = erf(sqrt u)/sqrt u,
+// g and h being entire functions of u.  `force` and `energy` hold alpha^3 g
+// and -alpha h as polynomials in v = scale r^2 - 1 (power basis, f32).
+#define MD_EWALD_TERMS 13
+struct md_ewald_poly {
+  int ok;  // the f32 evaluation meets the bound on [0, cutoff]
+  float scale;
+  float force[MD_EWALD_TERMS], energy[MD_EWALD_TERMS];
+  double force_err, energy_err;  // max |error| / (1/r^3), / (1/r), evaluated as the kernel does
+};
+md_ewald_poly md_ewald_fit(double alpha, double cutoff);  // nonbonded.hip
+
 struct md_nb_params {
   int nclusters, nslots;  // 8-atom clusters; slots in whole 64-atom super-clusters
   int ncells, gdim[3];    // sorting cells
   float cutoff2, list_cutoff2, half_skin2;
+  md_ewald_poly ewald;
 };
 
 // ---- launch groups ------------------------------------------------------------

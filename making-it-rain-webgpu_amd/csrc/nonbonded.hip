@@ -141,6 +141,12 @@ struct md_pair_args {
   // cutoff^2 (1 + 1e-5) in 2^-52 nm^2: a block whose boxes are this far apart
   // has no pair within the cutoff
   float prune2;
+  // the Coulomb polynomials of md_ewald_fit in frame units (v = fit_scale r2
+  // - 1 with r2 in 2^-52 nm^2; fit_force carries 2^-78 and fit_energy 2^-26,
+  // the units of 1/r^3 and 1/r), used when ewald_fit is set
+  int ewald_fit;
+  float fit_scale;
+  float fit_force[MD_EWALD_TERMS], fit_energy[MD_EWALD_TERMS];
 };
 
 // Everything the nonbonded kernels of one replica read: its table entry.
@@ -919,26 +925,51 @@ __global__ void __launch_bounds__(64 * MD_MASK_GROUP) k_nb_j_fill(const md_nb_ar
 // units gives kJ/mol/nm) and the energy carries 2^26.  Scaling by powers of
 // two is exact in f32, so this is the nm evaluation bit for bit, without
 // its two unit conversions per pair.
-template <bool ENERGY>
+//
+// FIT: the Coulomb factor is 1/r^3 plus a degree-12 polynomial in r^2
+// (md_ewald_fit), twelve FMAs in place of exp, a reciprocal and the erfc
+// polynomial; otherwise erfc(z) = exp(-z^2) P(1/(1 + z/2)).  `top` is the
+// force polynomial's leading coefficient, which the caller keeps in a VGPR
+// (an FMA reads one scalar operand, and the first Horner step has two).
+//
+// The result is the pair's F/r when `inc`; on the other lanes it is
+// undefined (r2 may be 0 there) and the caller must not add it.
+template <int N>
+__device__ __forceinline__ float md_horner(float top, const float *c, float v) {
+  float p = top;
+#pragma unroll
+  for (int k = N - 2; k >= 0; --k) p = fmaf(p, v, c[k]);
+  return p;
+}
+
+template <bool ENERGY, bool FIT>
 __device__ __forceinline__ float md_pair(float r2, float qi, float si, float ei, float qj, float sj, float ej,
-                                         bool inc, const md_pair_args &A, float &en, int &count) {
+                                         bool inc, float top, const md_pair_args &A, float &en, int &count) {
   float rinv = __builtin_amdgcn_rsqf(r2);
   float rinv2 = rinv * rinv;
-  float r = r2 * rinv;
-  float ex = __builtin_amdgcn_exp2f(A.exp_scale * r2);  // exp(-alpha^2 r^2)
-  float t = __builtin_amdgcn_rcpf(fmaf(A.half_alpha, r, 1.0f));
-  float erfc_v = ex * md_erfcx_poly(t);
+  float coul, ecoul = 0.0f;  // F/r and E per charge product
+  if (FIT) {
+    const float v = fmaf(r2, A.fit_scale, -1.0f);
+    coul = fmaf(rinv, rinv2, md_horner<MD_EWALD_TERMS>(top, A.fit_force, v));
+    if (ENERGY) ecoul = rinv + md_horner<MD_EWALD_TERMS>(A.fit_energy[MD_EWALD_TERMS - 1], A.fit_energy, v);
+  } else {
+    float r = r2 * rinv;
+    float ex = __builtin_amdgcn_exp2f(A.exp_scale * r2);  // exp(-alpha^2 r^2)
+    float t = __builtin_amdgcn_rcpf(fmaf(A.half_alpha, r, 1.0f));
+    float erfc_v = ex * md_erfcx_poly(t);
+    coul = fmaf(erfc_v, rinv, A.beta * ex) * rinv2;
+    if (ENERGY) ecoul = erfc_v * rinv;
+  }
   float qq = qi * qj;
-  float fc = qq * fmaf(erfc_v, rinv, A.beta * ex) * rinv2;
   float sig = si + sj;
   float eps = ei * ej;  // 4 eps_ij
   float s2 = sig * sig * rinv2;
   float s6 = s2 * s2 * s2;
-  float s12 = s6 * s6;
-  float flj = eps * fmaf(12.0f, s12, -6.0f * s6) * rinv2;
-  float fs = inc ? fc + flj : 0.0f;
+  float e6 = eps * s6;
+  float flj = e6 * fmaf(12.0f, s6, -6.0f) * rinv2;  // 4 eps (12 s^12 - 6 s^6) / r^2
+  float fs = fmaf(qq, coul, flj);
   if (ENERGY) {
-    float e = fmaf(qq * erfc_v, rinv, eps * (s12 - s6));
+    float e = fmaf(qq, ecoul, e6 * (s6 - 1.0f));
     en += inc ? e : 0.0f;
     count += inc ? 1 : 0;
   }
@@ -1016,7 +1047,7 @@ __device__ __forceinline__ float md_box_gap(int ilo, int ihi, int jlo, int jhi) 
 // Q32.32 at its end, then summed as integers over the segment and added once:
 // the rounding never depends on the segment length, so every launch group
 // gives the same bits whatever length it picked.
-template <bool ENERGY>
+template <bool ENERGY, bool FIT>
 __device__ __forceinline__ void md_pair_segment(const md_pair_args &A, int seg, int lane, int il, int jl,
                                                 int64_t &efix, int &count, int &blocks) {
   const int4 sg = md_uniform_xyz(A.segments + seg);
@@ -1052,6 +1083,10 @@ __device__ __forceinline__ void md_pair_segment(const md_pair_args &A, int seg, 
     ilo = A.cluster_lo[S * 8 + lane];
     ihi = A.cluster_hi[S * 8 + lane];
   }
+  // the force polynomial's leading coefficient in a VGPR (an empty asm
+  // with a VGPR operand puts it there once per segment)
+  float top = A.fit_force[MD_EWALD_TERMS - 1];
+  if (FIT) asm volatile("" : "+v"(top));
   for (int e = eb; e < ee; ++e) {
     const int J = md_uniform(A.entry_j + e);
     const int meta = md_uniform(A.entry_imask + e);
@@ -1099,14 +1134,18 @@ __device__ __forceinline__ void md_pair_segment(const md_pair_args &A, int seg, 
           // a block whose 64 pairs all lie beyond the cutoff (or are masked)
           // adds exactly zero: skip its arithmetic, wave-uniformly
           if (inc == 0) continue;
-          float fs = md_pair<ENERGY>(r2, qi[q], si[q], ei[q], qj, sj, ej,
-                                     __builtin_amdgcn_inverse_ballot_w64(inc), A, en, count);
-          fx[q] = fmaf(-fs, dx, fx[q]);
-          fy[q] = fmaf(-fs, dy, fy[q]);
-          fz[q] = fmaf(-fs, dz, fz[q]);
-          gx = fmaf(fs, dx, gx);
-          gy = fmaf(fs, dy, gy);
-          gz = fmaf(fs, dz, gz);
+          const bool counted = __builtin_amdgcn_inverse_ballot_w64(inc);
+          float fs = md_pair<ENERGY, FIT>(r2, qi[q], si[q], ei[q], qj, sj, ej, counted, top, A, en, count);
+          // the other lanes' fs is not a force: a divergent branch (the
+          // exec mask, scalar instructions) leaves them out without a select
+          if (counted) {
+            fx[q] = fmaf(-fs, dx, fx[q]);
+            fy[q] = fmaf(-fs, dy, fy[q]);
+            fz[q] = fmaf(-fs, dz, fz[q]);
+            gx = fmaf(fs, dx, gx);
+            gy = fmaf(fs, dy, gy);
+            gz = fmaf(fs, dz, gz);
+          }
         }
       }
       // j force: sum the 8 lanes that share j (lane%8), fixed order
@@ -1298,8 +1337,12 @@ __device__ __forceinline__ void md_pair_segment_compact(const md_pair_args &A, i
         const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
         float en = 0.0f;
         int cnt = 0;
-        L.fs[b][q * 64 + pl] = md_pair<false>(r2, __int_as_float(I.w), Ise.x, Ise.y, __int_as_float(Jd.w), Jse.x,
-                                              Jse.y, true, A, en, cnt);
+        const float qi = __int_as_float(I.w), qj = __int_as_float(Jd.w);
+        L.fs[b][q * 64 + pl] = A.ewald_fit
+                                   ? md_pair<false, true>(r2, qi, Ise.x, Ise.y, qj, Jse.x, Jse.y, true,
+                                                          A.fit_force[MD_EWALD_TERMS - 1], A, en, cnt)
+                                   : md_pair<false, false>(r2, qi, Ise.x, Ise.y, qj, Jse.x, Jse.y, true, 0.0f, A,
+                                                           en, cnt);
       }
     }
     __syncthreads();
@@ -1411,7 +1454,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
     const md_pair_args &A = T[md_locate(seg_map, g, seg)].P;
     int64_t efix = 0;
     int count = 0, blocks = 0;
-    md_pair_segment<ENERGY>(A, seg, lane, il, jl, efix, count, blocks);
+    if (A.ewald_fit)
+      md_pair_segment<ENERGY, true>(A, seg, lane, il, jl, efix, count, blocks);
+    else
+      md_pair_segment<ENERGY, false>(A, seg, lane, il, jl, efix, count, blocks);
     if (ENERGY) {
       int64_t ef = md_wave_sum_i64(efix);
       int64_t cf = md_wave_sum_i64((int64_t)count);
@@ -1528,6 +1574,105 @@ static md_nb_buffers &nbuf(md_system *s) {
   return *reinterpret_cast<md_nb_buffers *>(s->nb_ext);
 }
 
+// ---- the pair kernel's Coulomb polynomials ------------------------------------
+
+namespace {
+// g(u) = ((2/sqrt(pi)) e^-u - erf(sqrt u)/sqrt u) / u: below u = 1/2 by its
+// Taylor series, (2/sqrt(pi)) sum_{n>=1} (-1)^n u^(n-1)/n! 2n/(2n+1), whose
+// terms past n = 25 are below 1e-40
+double md_ewald_g(double u) {
+  const double sp = 2.0 / std::sqrt(M_PI);
+  if (u < 0.5) {
+    double term = -1.0, acc = 0.0;  // (-1)^n u^(n-1) / n!
+    for (int n = 1; n <= 25; ++n) {
+      if (n > 1) term *= -u / n;
+      acc += term * (2.0 * n) / (2.0 * n + 1.0);
+    }
+    return sp * acc;
+  }
+  const double r = std::sqrt(u);
+  return (sp * std::exp(-u) - std::erf(r) / r) / u;
+}
+
+// h(u) = erf(sqrt u)/sqrt u = (2/sqrt(pi)) sum_{n>=0} (-u)^n / (n! (2n+1))
+double md_ewald_h(double u) {
+  const double sp = 2.0 / std::sqrt(M_PI);
+  if (u < 0.5) {
+    double term = 1.0, acc = 0.0;
+    for (int n = 0; n <= 25; ++n) {
+      if (n > 0) term *= -u / n;
+      acc += term / (2.0 * n + 1.0);
+    }
+    return sp * acc;
+  }
+  const double r = std::sqrt(u);
+  return std::erf(r) / r;
+}
+
+float md_horner_host(const float *c, float v) {
+  float p = c[MD_EWALD_TERMS - 1];
+  for (int k = MD_EWALD_TERMS - 2; k >= 0; --k) p = std::fmaf(p, v, c[k]);
+  return p;
+}
+}  // namespace
+
+// Truncated Chebyshev series (from the 64-node interpolant on [0, U],
+// U = alpha^2 rc^2 (1 + 1e-4)) of degree MD_EWALD_TERMS - 1, rewritten in
+// powers of v = 2 r^2 / (rc^2 (1 + 1e-4)) - 1, coefficients rounded to f32.
+// The fit is then evaluated the way the pair kernel does (v by one fma from
+// the f32 r^2, Horner by fma) at 4,001 points of [0, rc] and compared with g
+// and h in double; it is used only if the force error stays within 1e-6 of
+// the bare 1/r^3 and the energy error within 1e-6 of 1/r.  That holds for
+// alpha rc up to ~3 (Ewald tolerances down to ~1e-4, e.g. 2.63 for 5e-4);
+// past it the kernel evaluates erfc and exp instead.
+md_ewald_poly md_ewald_fit(double alpha, double cutoff) {
+  md_ewald_poly P{};
+  const int N = 64, D = MD_EWALD_TERMS - 1;
+  const double rc2 = cutoff * cutoff * (1.0 + 1e-4), U = alpha * alpha * rc2;
+  double cheb[2][MD_EWALD_TERMS] = {};
+  for (int j = 0; j < N; ++j) {
+    const double th = M_PI * (j + 0.5) / N, u = 0.5 * U * (1.0 + std::cos(th));
+    const double fg = md_ewald_g(u), fh = md_ewald_h(u);
+    for (int k = 0; k <= D; ++k) {
+      const double w = (k == 0 ? 1.0 : 2.0) / N * std::cos(k * th);
+      cheb[0][k] += w * fg;
+      cheb[1][k] += w * fh;
+    }
+  }
+  // T_k(v) in powers of v by T_{k+1} = 2 v T_k - T_{k-1}
+  double t0[MD_EWALD_TERMS] = {1.0}, t1[MD_EWALD_TERMS] = {0.0, 1.0}, pw[2][MD_EWALD_TERMS] = {};
+  for (int k = 0; k <= D; ++k) {
+    const double *tk = k == 0 ? t0 : t1;
+    for (int m = 0; m <= D; ++m) {
+      pw[0][m] += cheb[0][k] * tk[m];
+      pw[1][m] += cheb[1][k] * tk[m];
+    }
+    if (k >= 1) {
+      double t2[MD_EWALD_TERMS] = {};
+      for (int m = 0; m <= D; ++m) t2[m] = (m > 0 ? 2.0 * t1[m - 1] : 0.0) - t0[m];
+      std::copy(t1, t1 + MD_EWALD_TERMS, t0);
+      std::copy(t2, t2 + MD_EWALD_TERMS, t1);
+    }
+  }
+  P.scale = (float)(2.0 / rc2);
+  for (int m = 0; m <= D; ++m) {
+    P.force[m] = (float)(pw[0][m] * alpha * alpha * alpha);
+    P.energy[m] = (float)(-pw[1][m] * alpha);
+  }
+  const int M = 4000;
+  for (int i = 1; i <= M; ++i) {
+    const float r2f = (float)(cutoff * cutoff * i / M);
+    const double r2 = r2f, r = std::sqrt(r2), u = alpha * alpha * r2;
+    const float v = std::fmaf(r2f, P.scale, -1.0f);
+    const double ef = std::fabs(md_horner_host(P.force, v) - alpha * alpha * alpha * md_ewald_g(u)) * r2 * r;
+    const double ee = std::fabs(md_horner_host(P.energy, v) + alpha * md_ewald_h(u)) * r;
+    P.force_err = std::max(P.force_err, ef);
+    P.energy_err = std::max(P.energy_err, ee);
+  }
+  P.ok = P.force_err <= 1e-6 && P.energy_err <= 1e-6;
+  return P;
+}
+
 void md_nb_setup(md_system *s) {
   auto &nb = s->nb;
   s->nb_ext = new md_nb_buffers();
@@ -1539,6 +1684,7 @@ void md_nb_setup(md_system *s) {
   double rlist = s->cutoff + skin;
   nb.cutoff2 = (float)(s->cutoff * s->cutoff);
   nb.list_cutoff2 = (float)(rlist * rlist);
+  nb.ewald = md_ewald_fit(s->alpha, s->cutoff);
   nb.half_skin2 = (float)(0.25 * skin * skin);
   B.list_cutoff = (float)rlist;
   // sorting cells ~0.3 nm along each fractional axis
@@ -1734,6 +1880,13 @@ static md_nb_args nb_args(md_system *s, int seg_len) {
   P.half_alpha = std::ldexp(0.5f * (float)s->alpha, -26);
   P.beta = std::ldexp((float)(2.0 * s->alpha / std::sqrt(M_PI)), -26);
   P.exp_scale = std::ldexp((float)(-s->alpha * s->alpha / std::log(2.0)), -52);
+  const md_ewald_poly &W = s->nb.ewald;
+  P.ewald_fit = W.ok;
+  P.fit_scale = std::ldexp(W.scale, -52);
+  for (int k = 0; k < MD_EWALD_TERMS; ++k) {
+    P.fit_force[k] = std::ldexp(W.force[k], -78);
+    P.fit_energy[k] = std::ldexp(W.energy[k], -26);
+  }
   a.pos = s->pos;
   a.cell_rank = s->cell_rank;
   a.cell_count = s->cell_count;

@@ -178,6 +178,8 @@ def library():
         L.md_constrain_velocities.argtypes = [C.c_void_p, C.c_double]
         L.md_get_mesh.argtypes = [C.c_void_p, _ip]
         L.md_probe_cell_order.argtypes = [_ip, _ip]
+        L.md_probe_ewald_fit.argtypes = [C.c_double, C.c_double, P(C.c_float), P(C.c_float), P(C.c_float),
+                                         P(C.c_int32), _dp, _dp]
         L.md_batch_create.argtypes = [P(C.c_void_p), C.c_int32, P(C.c_void_p)]
         L.md_batch_destroy.argtypes = [C.c_void_p]
         L.md_batch_destroy.restype = None
@@ -507,6 +509,23 @@ def probe_cell_order(dims):
     cells = np.zeros(int(np.prod(d)), dtype=np.int32)
     check(library().md_probe_cell_order(_i(d), _i(cells)))
     return cells
+
+
+def probe_ewald_fit(alpha: float, cutoff: float):
+    """The pair kernel's Coulomb polynomials for ``alpha`` (1/nm) and
+    ``cutoff`` (nm), host only: a dict with ``scale``, the 13 f32 ``force``
+    and ``energy`` coefficients, ``ok`` and the fit's ``force_err`` and
+    ``energy_err`` (relative to 1/r^3 and 1/r)."""
+    scale = C.c_float()
+    force = np.zeros(13, dtype=np.float32)
+    energy = np.zeros(13, dtype=np.float32)
+    ok = C.c_int32()
+    ferr, eerr = C.c_double(), C.c_double()
+    fp = C.POINTER(C.c_float)
+    check(library().md_probe_ewald_fit(alpha, cutoff, C.byref(scale), force.ctypes.data_as(fp),
+                                       energy.ctypes.data_as(fp), C.byref(ok), C.byref(ferr), C.byref(eerr)))
+    return {"scale": np.float32(scale.value), "force": force, "energy": energy, "ok": bool(ok.value),
+            "force_err": ferr.value, "energy_err": eerr.value}
 
 
 def probe_thermostat(seed: int, step: int, n: int):

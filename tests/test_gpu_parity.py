@@ -137,6 +137,27 @@ def test_config0_forces_and_groups(c0):
     s.close()
 
 
+def test_tight_ewald_tolerance_takes_the_erfc_path(c0):
+    """At Ewald tolerance 1e-6 (alpha rc = 3.7) the pair kernel's Coulomb
+    polynomial misses its bound and the kernel evaluates erfc and exp
+    instead; at the configurations' 5e-4 it takes the polynomial.  Both
+    paths meet the same gates against the oracle."""
+    import copy
+    assert native.probe_ewald_fit(O.OracleSystem(c0.system).alpha(), 1.0)["ok"]
+    b = copy.deepcopy(c0)
+    b.system.force("NonbondedForce").ewaldTolerance = 1e-6
+    o = O.OracleSystem(b.system)
+    assert o.alpha() > 3.6
+    assert not native.probe_ewald_fit(o.alpha(), 1.0)["ok"]
+    s, (total, groups, forces), (otot, og, of), err = _compare(b, mesh=(32, 32, 32))
+    print("tolerance 1e-6: force rel RMS", rel_rms(forces, of), "edge-aware", err, "groups", groups, og)
+    assert err < FORCE_RMS
+    for k in range(6):
+        scale = max(abs(og[k]), 1e-6 * abs(otot), 1.0)
+        assert abs(groups[k] - og[k]) / scale < ENERGY_REL, (k, groups[k], og[k])
+    s.close()
+
+
 def test_config1_triclinic_forces_and_groups(c1):
     assert tuple(c1.system.force("NonbondedForce").grid) == (64, 64, 64)
     s, (total, groups, forces), (otot, og, of), err = _compare(c1)
