@@ -49,11 +49,12 @@ import numpy as np  # noqa: E402
 FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
 PAIR_FLOPS = 76            # algorithmic FP32 flops per in-cutoff pair
-# HBM bytes per launch of the batched pair kernel on this ensemble: FETCH_SIZE +
-# WRITE_SIZE from two rocprofv3 --pmc passes over `bench.py --traffic-probe 3`,
+# HBM bytes per launch of the batched pair kernel on this ensemble with the
+# 64-entry work units the autotuner picks: FETCH_SIZE + WRITE_SIZE from two
+# rocprofv3 --pmc passes over `bench.py --traffic-probe 3 --pair-segment 64`,
 # each scaled by its reading for a 256 MiB device copy
 # (profiles/k_nb_pairs_ensemble64_traffic.md)
-PAIR_TRAFFIC_BYTES = 674.9e6
+PAIR_TRAFFIC_BYTES = 463.4e6
 DT_PS = 0.002
 METRIC = "aggregate ns/day at 2 fs (batched+learned vs plain, same GPU, fixed force error)"
 

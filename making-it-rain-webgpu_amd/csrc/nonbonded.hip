@@ -801,9 +801,13 @@ __global__ void __launch_bounds__(64 * MD_MASK_GROUP) k_nb_masks(const md_nb_arg
   }
 }
 
+// the pair kernels' segment counters, one per XCD (md_seg_queue)
+#define MD_XCDS 8
+#define MD_CURSOR_STRIDE 16  // ints between two counters: one 64-byte line
+
 // Per step: each slot's position as f32 in its super-cluster's frame.
 // The launch's first thread also lays out the pair kernel's work: the start of
-// each replica's list segments in one numbering, and the counter the pair
+// each replica's list segments in one numbering, and the counters the pair
 // kernel's waves take segments from, reset to 0.
 __global__ void k_nb_pack(const md_nb_args *__restrict__ T, md_bmap map, int *seg_start, int *cursor) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -813,7 +817,7 @@ __global__ void k_nb_pack(const md_nb_args *__restrict__ T, md_bmap map, int *se
       run += T[r].totals[1];
     }
     seg_start[map.R] = run;
-    *cursor = 0;
+    for (int c = 0; c < MD_XCDS; ++c) cursor[c * MD_CURSOR_STRIDE] = 0;
   }
   int blk;
   const md_nb_args &A = md_nb_entry(T, map, blk);
@@ -1412,16 +1416,56 @@ __device__ __forceinline__ void md_pair_segment_compact(const md_pair_args &A, i
   md_atomic_add_i64(&A.slot_force[2 * A.nslots + is], az);
 }
 
+// The pair kernels' work queue.  The segments, numbered replica by replica
+// and in list (Hilbert) order within each, are cut into MD_XCDS contiguous
+// ranges with a counter each (one 64-byte line apart).  A workgroup takes
+// segments from the range of the XCD it runs on first -- the dispatcher deals
+// workgroups to the XCDs in turn, workgroup b to XCD b mod 8 -- then from the
+// following ranges once that one is empty.  Each XCD's waves thus walk one
+// neighbourhood of the list at a time and its L2 keeps that neighbourhood's
+// j clusters, instead of all eight L2s fetching every cluster.  Which wave
+// takes which segment never changes a result (each segment's sums are
+// rounded to Q32.32 on their own).
+//
+// Every lane joins the atomic (lane 0 adds 1, the others 0) and readfirstlane
+// makes the index wave-uniform, so the exits are scalar branches.  A
+// lane-0-only atomic followed by a broadcast is not safe here: the optimiser
+// threads the per-segment `lane == 0` energy flush into the next `lane == 0`
+// atomic, after which lanes 1-63 loop back to the broadcast without lane 0
+// and never leave.  The counters only grow and a wave leaves a range at its
+// first index past the end, so every wave leaves.
+struct md_seg_queue {
+  int *cursor;
+  int nseg, home, range, lo, hi, lane;
+  __device__ md_seg_queue(int *c, int n, int l) : cursor(c), nseg(n), home((int)(blockIdx.x & (MD_XCDS - 1))),
+                                                  range(0), lane(l) {
+    bounds();
+  }
+  __device__ void bounds() {
+    const int c = (home + range) & (MD_XCDS - 1);
+    lo = (int)(((int64_t)nseg * c) / MD_XCDS);
+    hi = (int)(((int64_t)nseg * (c + 1)) / MD_XCDS);
+  }
+  __device__ bool next(int &g) {
+    while (range < MD_XCDS) {
+      const int c = (home + range) & (MD_XCDS - 1);
+      g = lo + __builtin_amdgcn_readfirstlane(atomicAdd(cursor + c * MD_CURSOR_STRIDE, lane == 0 ? 1 : 0));
+      if (g < hi) return true;
+      if (++range < MD_XCDS) bounds();
+    }
+    return false;
+  }
+};
+
 // The compacted schedule's kernel (forces only; energies use the direct one).
 template <bool BATCH>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_nb_pairs_compact(const md_nb_args *__restrict__ T, md_bmap seg_map, int *cursor) {
   __shared__ md_pair_lds L;
   const int nseg = seg_map.start[seg_map.R];
   const int lane = threadIdx.x, il = lane >> 3, jl = lane & 7;
-  for (;;) {
-    // as in k_nb_pairs: every lane in the atomic, the index made uniform
-    const int g = __builtin_amdgcn_readfirstlane(atomicAdd(cursor, lane == 0 ? 1 : 0));
-    if (g >= nseg) break;
+  md_seg_queue Q(cursor, nseg, lane);
+  int g;
+  while (Q.next(g)) {
     int seg;
     const md_pair_args &A = T[md_locate(seg_map, g, seg)].P;
     md_pair_segment_compact(A, seg, lane, il, jl, L);
@@ -1436,20 +1480,13 @@ template <bool ENERGY, bool BATCH = false>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_nb_pairs(const md_nb_args *__restrict__ T, md_bmap seg_map, int *cursor) {
   const int nseg = seg_map.start[seg_map.R];
   const int lane = threadIdx.x, il = lane >> 3, jl = lane & 7;
-  // A resident grid of waves takes segments from one counter until none are
+  // A resident grid of waves takes segments from the queue until none are
   // left, across every replica's list.  Each segment's energy is rounded to
   // Q32.32 and added on its own, so the sums do not depend on which wave took
   // which segment.
-  //
-  // The segment number is taken with every lane in the atomic (lane 0 adds 1,
-  // the others 0) and made wave-uniform with readfirstlane, so the loop's exit
-  // is a scalar branch.  A lane-0-only atomic followed by a __shfl broadcast is
-  // not safe here: the optimiser threads the per-segment `lane == 0` energy
-  // flush below into the next iteration's `lane == 0` atomic, after which
-  // lanes 1-63 loop back to the broadcast without lane 0 and never leave.
-  for (;;) {
-    const int g = __builtin_amdgcn_readfirstlane(atomicAdd(cursor, lane == 0 ? 1 : 0));
-    if (g >= nseg) break;
+  md_seg_queue Q(cursor, nseg, lane);
+  int g;
+  while (Q.next(g)) {
     int seg;
     const md_pair_args &A = T[md_locate(seg_map, g, seg)].P;
     int64_t efix = 0;
@@ -1954,7 +1991,7 @@ void md_nb_group_setup(md_group *g) {
   G->slots = md_make_launch(g, slots);
   G->seg.map.start = md_group_alloc<int>(g, R + 1);
   G->seg.map.R = R;
-  G->cursor = md_group_alloc<int>(g, 1);
+  G->cursor = md_group_alloc<int>(g, MD_XCDS * MD_CURSOR_STRIDE);
   G->table = md_group_alloc<md_nb_args>(g, R);
   // the pair kernel keeps one resident grid busy from the segment counter
   int cus = 1, per_cu[2] = {1, 1};
