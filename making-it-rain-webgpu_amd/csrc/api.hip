@@ -554,6 +554,27 @@ static void create(md_system *s, const md_system_desc *d, const md_run_options *
   s->ctl = md_alloc<md_control>(s, 1);
   s->term_atoms = upload(s, ta);
   s->term_param = upload(s, tp);
+  {
+    // the atom of each force piece, in the order k_terms numbers them
+    std::vector<int> owner;
+    const int kinds[5] = {s->n_bonds, s->n_angles, s->n_torsions, s->n_exceptions, s->n_restraints};
+    const int per[5] = {2, 3, 4, 2, 1};
+    size_t t = 0;
+    for (int k = 0; k < 5; ++k)
+      for (int i = 0; i < kinds[k]; ++i, ++t) {
+        const int at[4] = {ta[t].x, ta[t].y, ta[t].z, ta[t].w};
+        for (int j = 0; j < per[k]; ++j) owner.push_back(at[j]);
+      }
+    std::vector<int> start(np + 1, 0), list(owner.size());
+    for (int a : owner) ++start[a + 1];
+    for (int a = 0; a < np; ++a) start[a + 1] += start[a];
+    std::vector<int> fill(start.begin(), start.end() - 1);
+    for (size_t q = 0; q < owner.size(); ++q) list[fill[owner[q]]++] = (int)q;
+    s->n_pieces = (int)owner.size();
+    s->term_piece = md_alloc<int64_t>(s, std::max<size_t>(1, 3 * owner.size()));
+    s->atom_piece_start = upload(s, start);
+    s->atom_piece = upload(s, list);
+  }
   s->restraint_ref = upload(s, rref);
   s->water_atoms = upload(s, R.waters);
   s->water_dist = upload(s, R.water_d);

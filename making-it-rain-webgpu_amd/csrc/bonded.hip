@@ -27,6 +27,8 @@ struct md_term_args {
   const int64_t *pos;
   int64_t *force;
   int64_t *energy;
+  int64_t *piece;  // [pieces][3]: each (term, atom) force in Q32.32
+  const int *piece_start, *piece_list;  // by atom: [np + 1], then piece numbers
   int np;
   int n_bonds, n_angles, n_torsions, n_exceptions, n_restraints;
   double alpha;
@@ -35,10 +37,31 @@ struct md_term_args {
 
 __device__ __forceinline__ int64_t md_d_to_fixed(double v) { return (int64_t)rint(v * 4294967296.0); }
 
-__device__ __forceinline__ void md_add_force(const md_term_args &A, int atom, double fx, double fy, double fz) {
-  md_atomic_add_i64(&A.force[atom], md_d_to_fixed(fx));
-  md_atomic_add_i64(&A.force[A.np + atom], md_d_to_fixed(fy));
-  md_atomic_add_i64(&A.force[2 * A.np + atom], md_d_to_fixed(fz));
+// Piece p of the term list takes one atom's force from one term, stored
+// plainly: atoms shared by many terms (a torsion's four atoms sit in a dozen
+// other terms) would otherwise serialise their 64-bit atomics at memory.
+__device__ __forceinline__ void md_put_piece(const md_term_args &A, int p, double fx, double fy, double fz) {
+  int64_t *q = A.piece + 3 * (size_t)p;
+  q[0] = md_d_to_fixed(fx);
+  q[1] = md_d_to_fixed(fy);
+  q[2] = md_d_to_fixed(fz);
+}
+
+// The first piece of term t: the terms are ordered bonds, angles, torsions,
+// exceptions, restraints, with 2, 3, 4, 2 and 1 atoms, pieces in that order.
+__device__ __forceinline__ int md_first_piece(const md_term_args &A, int t) {
+  int p = 0, k = t;
+  if (k < A.n_bonds) return 2 * k;
+  p += 2 * A.n_bonds;
+  k -= A.n_bonds;
+  if (k < A.n_angles) return p + 3 * k;
+  p += 3 * A.n_angles;
+  k -= A.n_angles;
+  if (k < A.n_torsions) return p + 4 * k;
+  p += 4 * A.n_torsions;
+  k -= A.n_torsions;
+  if (k < A.n_exceptions) return p + 2 * k;
+  return p + 2 * A.n_exceptions + (k - A.n_exceptions);
 }
 
 __device__ __forceinline__ void md_diff(const md_term_args &A, int from, int to, double *d) {
@@ -58,6 +81,7 @@ __global__ void __launch_bounds__(256) k_terms(const md_term_args *__restrict__ 
   if (t < total) {
     int4 at = A.atoms[t];
     double4 p = A.param[t];
+    const int p0 = md_first_piece(A, t);
     if (t < A.n_bonds) {
       double d[3];
       md_diff(A, at.x, at.y, d);
@@ -65,8 +89,8 @@ __global__ void __launch_bounds__(256) k_terms(const md_term_args *__restrict__ 
       double dr = r - p.x;
       e_bond = 0.5 * p.y * dr * dr;
       double sc = p.y * dr / r;
-      md_add_force(A, at.x, sc * d[0], sc * d[1], sc * d[2]);
-      md_add_force(A, at.y, -sc * d[0], -sc * d[1], -sc * d[2]);
+      md_put_piece(A, p0, sc * d[0], sc * d[1], sc * d[2]);
+      md_put_piece(A, p0 + 1, -sc * d[0], -sc * d[1], -sc * d[2]);
     } else if (t < A.n_bonds + A.n_angles) {
       double v1[3], v2[3];
       md_diff(A, at.y, at.x, v1);
@@ -85,9 +109,9 @@ __global__ void __launch_bounds__(256) k_terms(const md_term_args *__restrict__ 
                       -ta * (v1[0] * n[1] - v1[1] * n[0])};
       double f3[3] = {-tc * (v2[1] * n[2] - v2[2] * n[1]), -tc * (v2[2] * n[0] - v2[0] * n[2]),
                       -tc * (v2[0] * n[1] - v2[1] * n[0])};
-      md_add_force(A, at.x, f1[0], f1[1], f1[2]);
-      md_add_force(A, at.z, f3[0], f3[1], f3[2]);
-      md_add_force(A, at.y, -f1[0] - f3[0], -f1[1] - f3[1], -f1[2] - f3[2]);
+      md_put_piece(A, p0, f1[0], f1[1], f1[2]);
+      md_put_piece(A, p0 + 1, -f1[0] - f3[0], -f1[1] - f3[1], -f1[2] - f3[2]);
+      md_put_piece(A, p0 + 2, f3[0], f3[1], f3[2]);
     } else if (t < A.n_bonds + A.n_angles + A.n_torsions) {
       double b1[3], b2[3], b3[3];
       md_diff(A, at.x, at.y, b1);
@@ -106,9 +130,9 @@ __global__ void __launch_bounds__(256) k_terms(const md_term_args *__restrict__ 
       double per = p.z;
       e_torsion = p.y * (1.0 + cos(per * phi - p.x));
       double dedphi = -p.y * per * sin(per * phi - p.x);
+      double f1[3] = {0, 0, 0}, f4[3] = {0, 0, 0}, f2[3] = {0, 0, 0}, f3[3] = {0, 0, 0};
       if (n1sq != 0.0 && n2sq != 0.0) {
         double s1 = dedphi * b2n / n1sq, s4 = -dedphi * b2n / n2sq;
-        double f1[3], f4[3], f2[3], f3[3];
         double b2sq = b2n * b2n;
         double ss = (b1[0] * b2[0] + b1[1] * b2[1] + b1[2] * b2[2]) / b2sq;
         double tt = (b3[0] * b2[0] + b3[1] * b2[1] + b3[2] * b2[2]) / b2sq;
@@ -118,11 +142,11 @@ __global__ void __launch_bounds__(256) k_terms(const md_term_args *__restrict__ 
           f2[k] = -(1.0 + ss) * f1[k] + tt * f4[k];
           f3[k] = ss * f1[k] - (1.0 + tt) * f4[k];
         }
-        md_add_force(A, at.x, f1[0], f1[1], f1[2]);
-        md_add_force(A, at.y, f2[0], f2[1], f2[2]);
-        md_add_force(A, at.z, f3[0], f3[1], f3[2]);
-        md_add_force(A, at.w, f4[0], f4[1], f4[2]);
       }
+      md_put_piece(A, p0, f1[0], f1[1], f1[2]);
+      md_put_piece(A, p0 + 1, f2[0], f2[1], f2[2]);
+      md_put_piece(A, p0 + 2, f3[0], f3[1], f3[2]);
+      md_put_piece(A, p0 + 3, f4[0], f4[1], f4[2]);
     } else if (t < A.n_bonds + A.n_angles + A.n_torsions + A.n_exceptions) {
       // param = (K q_i q_j, K chargeProd, sigma, epsilon)
       int64_t dx = A.pos[at.y] - A.pos[at.x];
@@ -151,11 +175,10 @@ __global__ void __launch_bounds__(256) k_terms(const md_term_args *__restrict__ 
         e_exc += 4.0 * p.w * (s12 - s6);
         dedr += (4.0 * p.w / r) * (6.0 * s6 - 12.0 * s12);
       }
-      if (dedr != 0.0) {
-        double sc = dedr / r;
-        md_add_force(A, at.x, sc * d[0], sc * d[1], sc * d[2]);
-        md_add_force(A, at.y, -sc * d[0], -sc * d[1], -sc * d[2]);
-      }
+      // every piece is written at every evaluation (zero where the term adds nothing)
+      const double sc = dedr != 0.0 ? dedr / r : 0.0;
+      md_put_piece(A, p0, sc * d[0], sc * d[1], sc * d[2]);
+      md_put_piece(A, p0 + 1, -sc * d[0], -sc * d[1], -sc * d[2]);
     } else {
       int k = t - (A.n_bonds + A.n_angles + A.n_torsions + A.n_exceptions);
       int atom = at.x;
@@ -166,7 +189,7 @@ __global__ void __launch_bounds__(256) k_terms(const md_term_args *__restrict__ 
       const double s = 2.3283064365386963e-10;
       double d[3] = {(double)dx * s, (double)dy * s, (double)dz * s};
       e_res = p.x * (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-      md_add_force(A, atom, -2.0 * p.x * d[0], -2.0 * p.x * d[1], -2.0 * p.x * d[2]);
+      md_put_piece(A, p0, -2.0 * p.x * d[0], -2.0 * p.x * d[1], -2.0 * p.x * d[2]);
     }
   }
   if (ENERGY) {
@@ -178,6 +201,27 @@ __global__ void __launch_bounds__(256) k_terms(const md_term_args *__restrict__ 
       if ((threadIdx.x & 63) == 0 && w != 0) md_atomic_add_i64(A.energy + slot[k], w);
     }
   }
+}
+
+// Each atom's share of the terms: the sum of its pieces added to its force.
+// Kernels on one stream run in order and a thread is the only one holding
+// its atom here, so a plain read-add-write is enough; integer sums make the
+// order of the pieces irrelevant to the bits.
+__global__ void k_terms_gather(const md_term_args *__restrict__ T, md_bmap map) {
+  int blk;
+  const md_term_args &A = T[md_locate(map, blockIdx.x, blk)];
+  const int a = blk * blockDim.x + threadIdx.x;
+  if (a >= A.np) return;
+  const int b = A.piece_start[a], e = A.piece_start[a + 1];
+  if (b == e) return;
+  int64_t f[3] = {0, 0, 0};
+  for (int k = b; k < e; ++k) {
+    const int64_t *q = A.piece + 3 * (size_t)A.piece_list[k];
+    f[0] += q[0];
+    f[1] += q[1];
+    f[2] += q[2];
+  }
+  for (int d = 0; d < 3; ++d) A.force[d * A.np + a] += f[d];
 }
 
 __global__ void k_zero_forces(const md_term_args *__restrict__ T, md_bmap map, int zero_energy) {
@@ -195,6 +239,9 @@ static md_term_args term_args(md_system *s) {
   A.restraint_ref = s->restraint_ref;
   A.pos = s->pos;
   A.force = s->force;
+  A.piece = s->term_piece;
+  A.piece_start = s->atom_piece_start;
+  A.piece_list = s->atom_piece;
   A.energy = s->energy;
   A.np = s->np;
   A.n_bonds = s->n_bonds;
@@ -209,7 +256,7 @@ static md_term_args term_args(md_system *s) {
 
 struct md_bonded_group {
   md_term_args *table = nullptr;
-  md_launch zero, terms;
+  md_launch zero, terms, atoms;
 };
 
 static md_bonded_group &bg(md_group *g) { return *reinterpret_cast<md_bonded_group *>(g->bonded); }
@@ -217,14 +264,16 @@ static md_bonded_group &bg(md_group *g) { return *reinterpret_cast<md_bonded_gro
 void md_bonded_group_setup(md_group *g) {
   auto *G = new md_bonded_group();
   g->bonded = G;
-  std::vector<int> zero, terms;
+  std::vector<int> zero, terms, atoms;
   for (md_system *s : g->reps) {
     zero.push_back((3 * s->np + 255) / 256);
+    atoms.push_back(s->n_pieces > 0 ? (s->np + 255) / 256 : 0);
     int total = s->n_bonds + s->n_angles + s->n_torsions + s->n_exceptions + s->n_restraints;
     terms.push_back((total + 255) / 256);
   }
   G->zero = md_make_launch(g, zero);
   G->terms = md_make_launch(g, terms);
+  G->atoms = md_make_launch(g, atoms);
   G->table = md_group_alloc<md_term_args>(g, g->reps.size());
   md_bonded_group_refresh(g);
 }
@@ -268,5 +317,6 @@ void md_bonded(md_group *g, bool energy) {
     k_terms<true><<<G.terms.total, 256, 0, g->stream>>>(G.table, G.terms.map);
   else
     k_terms<false><<<G.terms.total, 256, 0, g->stream>>>(G.table, G.terms.map);
+  k_terms_gather<<<G.atoms.total, 256, 0, g->stream>>>(G.table, G.atoms.map);
   MD_HIP(hipGetLastError());
 }

@@ -255,6 +255,14 @@ struct md_system {
   int n_bonds = 0, n_angles = 0, n_torsions = 0, n_exceptions = 0, n_restraints = 0;
   int4 *term_atoms = nullptr;
   double4 *term_param = nullptr;
+  // the terms' forces as pieces, one Q32.32 triple per (term, atom) in term
+  // order (2, 3, 4, 2, 1 atoms for a bond, angle, torsion, exception,
+  // restraint), and by atom the pieces to sum: k_terms stores pieces,
+  // k_terms_gather adds each atom's sum to its force (no atomics)
+  int n_pieces = 0;
+  int64_t *term_piece = nullptr;    // [n_pieces][3]
+  int *atom_piece_start = nullptr;  // [np + 1]
+  int *atom_piece = nullptr;        // [n_pieces]
   int64_t *restraint_ref = nullptr;  // [3][n_restraints] Q32.32
 
   // ---- PME ------------------------------------------------------------
@@ -383,6 +391,9 @@ struct md_min_work {
   int nc = 0;                 // live constraints (springs)
   int *p1 = nullptr, *p2 = nullptr;
   double *dist = nullptr;
+  double *spring_f = nullptr;  // [nc][3]: each spring's pull on its first atom
+  int *cstart = nullptr;       // [np + 1]: by atom, its springs in clist
+  int *clist = nullptr;        // spring c as c (first atom) or -c - 1 (second)
 };
 md_min_work *md_min_work_create(md_system *s, const std::vector<int> &p1, const std::vector<int> &p2,
                                 const std::vector<double> &dist);

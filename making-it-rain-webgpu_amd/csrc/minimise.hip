@@ -112,11 +112,12 @@ __global__ void k_min_gradient(const int64_t *force, const float *inv_mass, doub
   }
 }
 
-// Constraint springs (k/2)(r - d)^2 of OpenMM's minimiser objective: their
-// gradient onto g (frozen atoms excluded), their energy into partial[block].
-// One thread per constraint; the gradient adds are f64 atomics.
+// Constraint springs (k/2)(r - d)^2 of OpenMM's minimiser objective: each
+// spring's pull stored per spring (k_min_spring_gather adds them to the
+// atoms in a fixed order, so the gradient's bits never depend on thread
+// timing, as f64 atomics would make them), their energy into partial[block].
 __global__ void k_min_springs(const double *x, const int *p1, const int *p2, const double *dist, int nc,
-                              const float *inv_mass, double kspring, double *g, int np, double *partial) {
+                              double kspring, double *spring_f, int np, double *partial) {
   double e = 0.0;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
     const int a = p1[c], b = p2[c];
@@ -128,13 +129,26 @@ __global__ void k_min_springs(const double *x, const int *p1, const int *p2, con
     const double r = sqrt(r2), dr = r - dist[c];
     e += 0.5 * kspring * dr * dr;
     const double f = kspring * dr / (r == 0.0 ? 1.0 : r);
-    for (int d = 0; d < 3; ++d) {
-      if (inv_mass[a] != 0.0f) atomicAdd(&g[d * np + a], f * v[d]);
-      if (inv_mass[b] != 0.0f) atomicAdd(&g[d * np + b], -f * v[d]);
-    }
+    for (int d = 0; d < 3; ++d) spring_f[3 * (size_t)c + d] = f * v[d];
   }
   e = md_block_sum(e);
   if (threadIdx.x == 0) partial[blockIdx.x] = e;
+}
+
+// Each moving atom's springs, in the order of its list, onto its gradient.
+__global__ void k_min_spring_gather(const double *spring_f, const int *cstart, const int *clist,
+                                    const float *inv_mass, double *g, int n, int np) {
+  for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < n; a += gridDim.x * blockDim.x) {
+    if (inv_mass[a] == 0.0f || cstart[a] == cstart[a + 1]) continue;
+    double sum[3] = {0.0, 0.0, 0.0};
+    for (int k = cstart[a]; k < cstart[a + 1]; ++k) {
+      const int code = clist[k];
+      const int c = code >= 0 ? code : -code - 1;
+      const double sign = code >= 0 ? 1.0 : -1.0;
+      for (int d = 0; d < 3; ++d) sum[d] += sign * spring_f[3 * (size_t)c + d];
+    }
+    for (int d = 0; d < 3; ++d) g[d * np + a] += sum[d];
+  }
 }
 
 // the largest relative constraint deviation |r - d| / d, per block
@@ -168,6 +182,26 @@ md_min_work *md_min_work_create(md_system *s, const std::vector<int> &p1, const 
   w->p1 = md_alloc<int>(s, std::max<size_t>(1, p1.size()));
   w->p2 = md_alloc<int>(s, std::max<size_t>(1, p2.size()));
   w->dist = md_alloc<double>(s, std::max<size_t>(1, dist.size()));
+  // by atom, the springs that pull it (first atoms +c, second atoms -c - 1)
+  std::vector<int> cstart(s->np + 1, 0), clist(2 * p1.size());
+  for (size_t c = 0; c < p1.size(); ++c) {
+    ++cstart[p1[c] + 1];
+    ++cstart[p2[c] + 1];
+  }
+  for (int a = 0; a < s->np; ++a) cstart[a + 1] += cstart[a];
+  {
+    std::vector<int> fill(cstart.begin(), cstart.end() - 1);
+    for (size_t c = 0; c < p1.size(); ++c) {
+      clist[fill[p1[c]]++] = (int)c;
+      clist[fill[p2[c]]++] = -(int)c - 1;
+    }
+  }
+  w->spring_f = md_alloc<double>(s, std::max<size_t>(1, 3 * p1.size()));
+  w->cstart = md_alloc<int>(s, cstart.size());
+  w->clist = md_alloc<int>(s, std::max<size_t>(1, clist.size()));
+  MD_HIP(hipMemcpyAsync(w->cstart, cstart.data(), sizeof(int) * cstart.size(), hipMemcpyHostToDevice, s->stream));
+  if (!clist.empty())
+    MD_HIP(hipMemcpyAsync(w->clist, clist.data(), sizeof(int) * clist.size(), hipMemcpyHostToDevice, s->stream));
   if (w->nc) {
     MD_HIP(hipMemcpyAsync(w->p1, p1.data(), sizeof(int) * p1.size(), hipMemcpyHostToDevice, s->stream));
     MD_HIP(hipMemcpyAsync(w->p2, p2.data(), sizeof(int) * p2.size(), hipMemcpyHostToDevice, s->stream));
@@ -227,8 +261,10 @@ double md_min_gradient(md_system *s, md_min_work *w, const double *x, double ksp
   double spring = 0.0;
   if (w->nc) {
     const int nb = blocks_for(w->nc);
-    k_min_springs<<<nb, MD_MIN_BLOCK, 0, s->stream>>>(x, w->p1, w->p2, w->dist, w->nc, s->inv_mass, kspring, g, s->np,
+    k_min_springs<<<nb, MD_MIN_BLOCK, 0, s->stream>>>(x, w->p1, w->p2, w->dist, w->nc, kspring, w->spring_f, s->np,
                                                      w->partial);
+    k_min_spring_gather<<<blocks_for(s->n), MD_MIN_BLOCK, 0, s->stream>>>(w->spring_f, w->cstart, w->clist,
+                                                                          s->inv_mass, g, s->n, s->np);
     k_min_finish<<<1, 64, 0, s->stream>>>(w->partial, nb, 0, w->scalar);
     MD_HIP(hipGetLastError());
     spring = read_scalar(s, w);
