@@ -883,9 +883,25 @@ __global__ void k_nb_forces(const md_nb_args *__restrict__ T, md_bmap map) {
   }
   if (a < 0) return;
   const int J = s / MD_CLUSTER, q = s - J * MD_CLUSTER;
-  const float *jf = A.P.j_force;
-  for (int k = A.L.j_start[J], end = A.L.j_start[J + 1]; k < end; ++k) {
-    const float *e = jf + (size_t)A.L.j_entries[k] * 24 + q * 3;
+  const float *jf = A.P.j_force + q * 3;
+  const int *je = A.L.j_entries;
+  int k = A.L.j_start[J];
+  const int end = A.L.j_start[J + 1];
+  // four entries' records in flight at a time: each iteration's loads are
+  // independent of the last, so the loop waits once per four entries on the
+  // entry index and once on the records (integer sums: order-free)
+  for (; k + 4 <= end; k += 4) {
+    const int e0 = je[k], e1 = je[k + 1], e2 = je[k + 2], e3 = je[k + 3];
+    const float *r0 = jf + (size_t)e0 * 24, *r1 = jf + (size_t)e1 * 24, *r2 = jf + (size_t)e2 * 24,
+                *r3 = jf + (size_t)e3 * 24;
+    const float x0 = r0[0], y0 = r0[1], z0 = r0[2], x1 = r1[0], y1 = r1[1], z1 = r1[2];
+    const float x2 = r2[0], y2 = r2[1], z2 = r2[2], x3 = r3[0], y3 = r3[1], z3 = r3[2];
+    f[0] += md_to_fixed(x0) + md_to_fixed(x1) + md_to_fixed(x2) + md_to_fixed(x3);
+    f[1] += md_to_fixed(y0) + md_to_fixed(y1) + md_to_fixed(y2) + md_to_fixed(y3);
+    f[2] += md_to_fixed(z0) + md_to_fixed(z1) + md_to_fixed(z2) + md_to_fixed(z3);
+  }
+  for (; k < end; ++k) {
+    const float *e = jf + (size_t)je[k] * 24;
     f[0] += md_to_fixed(e[0]);
     f[1] += md_to_fixed(e[1]);
     f[2] += md_to_fixed(e[2]);
@@ -902,16 +918,17 @@ __global__ void __launch_bounds__(1024) k_nb_j_scan(const md_nb_args *__restrict
   md_block_scan(A.L.j_count, A.L.j_start, A.c.nclusters, A.L.j_count);
 }
 
-__global__ void __launch_bounds__(64 * MD_MASK_GROUP) k_nb_j_fill(const md_nb_args *__restrict__ T, md_bmap map) {
-  int blk;
-  const md_nb_args &A = md_nb_entry(T, map, blk);
+// One wave per super-cluster (its ~200 entries are a few passes of 64
+// lanes): a launch over replicas that did not rebuild then dispatches 16
+// times fewer waves that return at once.
+__global__ void __launch_bounds__(64) k_nb_j_fill(const md_nb_args *__restrict__ T, md_bmap map) {
+  int S;
+  const md_nb_args &A = md_nb_entry(T, map, S);
   if (!A.ctl->rebuild_active) return;
   const md_list_ptrs &L = A.L;
-  const int wave = blk * MD_MASK_GROUP + (int)(threadIdx.x >> 6);
-  const int S = wave / MD_MASK_WAVES, part = wave % MD_MASK_WAVES;
   if (S >= A.c.nsuper) return;
   const int eb = S * A.c.entries_per_super, ee = eb + min(L.super_count[S], A.c.entries_per_super);
-  for (int e = eb + part * 64 + (int)(threadIdx.x & 63); e < ee; e += MD_MASK_WAVES * 64) {
+  for (int e = eb + (int)threadIdx.x; e < ee; e += 64) {
     const int J = L.entry_j[e];
     L.j_entries[L.j_start[J] + atomicAdd(&L.j_count[J], 1)] = e;
   }
@@ -2077,7 +2094,7 @@ void md_nb_build(md_group *g) {
   md_mark(g, MD_PHASE_MASKS);
   k_nb_masks<<<G.masks.total, 64 * MD_MASK_GROUP, 0, st>>>(T, G.masks.map);
   k_nb_j_scan<<<G.scan.total, 1024, 0, st>>>(T, G.scan.map);
-  k_nb_j_fill<<<G.masks.total, 64 * MD_MASK_GROUP, 0, st>>>(T, G.masks.map);
+  k_nb_j_fill<<<G.supers.total, 64, 0, st>>>(T, G.supers.map);
   MD_HIP(hipGetLastError());
 }
 
