@@ -360,7 +360,14 @@ __global__ void k_nb_clusters(const md_nb_args *__restrict__ T, md_bmap map) {
     for (int d = 0; d < 3; ++d) cluster_center[d * c.nclusters + k] = cen[d];
     cluster_half[k] = make_float4(half[0], half[1], half[2], 0.0f);
     A.L.j_count[k] = 0;
-    atomicMax(&ctl->cluster_radius_bits, __float_as_uint(radius));
+  }
+  // the largest cluster radius (non-negative floats order as their bits):
+  // reduced over the wave first, one atomic per wave on the shared word
+  // rather than one per cluster, which all queue on one address
+  {
+    float rmax = live ? radius : -1.0f;
+    for (int off = 32; off > 0; off >>= 1) rmax = fmaxf(rmax, __shfl_xor(rmax, off, 64));
+    if (lane == 0 && rmax >= 0.0f) atomicMax(&ctl->cluster_radius_bits, __float_as_uint(rmax));
   }
   // super-cluster frame: the bounding box centre over its 8 clusters
   int64_t slo[3], shi[3];
