@@ -383,6 +383,7 @@ __global__ void k_nb_clusters(const md_nb_args *__restrict__ T, md_bmap map) {
   int s = k / 8;
   int64_t origin[3];
   for (int d = 0; d < 3; ++d) origin[d] = slo[d] <= shi[d] ? slo[d] + ((shi[d] - slo[d]) >> 1) : 0;
+  float srad = -1.0f;
   if (live && (lane & 7) == 0 && s < c.nsuper) {
     float r2 = 0.0f;
     for (int d = 0; d < 3; ++d) {
@@ -391,9 +392,12 @@ __global__ void k_nb_clusters(const md_nb_args *__restrict__ T, md_bmap map) {
       r2 += h * h;
     }
     // rounded up a little: the list test that uses it must stay conservative
-    super_radius[s] = sqrtf(r2) * 1.0001f + 1e-4f;
-    atomicMax(&ctl->super_radius_bits, __float_as_uint(super_radius[s]));
+    srad = sqrtf(r2) * 1.0001f + 1e-4f;
+    super_radius[s] = srad;
   }
+  // the largest super-cluster radius, one atomic per wave as above
+  for (int off = 32; off > 0; off >>= 1) srad = fmaxf(srad, __shfl_xor(srad, off, 64));
+  if (lane == 0 && srad >= 0.0f) atomicMax(&ctl->super_radius_bits, __float_as_uint(srad));
   if (!live) return;
   for (int q = 0; q < MD_CLUSTER; ++q) {
     int slot = k * MD_CLUSTER + q;
