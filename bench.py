@@ -54,7 +54,7 @@ PAIR_FLOPS = 76            # algorithmic FP32 flops per in-cutoff pair
 # rocprofv3 --pmc passes over `bench.py --traffic-probe 3 --pair-segment 64`,
 # each scaled by its reading for a 256 MiB device copy
 # (profiles/k_nb_pairs_ensemble64_traffic.md)
-PAIR_TRAFFIC_BYTES = 463.4e6
+PAIR_TRAFFIC_BYTES = 463.3e6
 DT_PS = 0.002
 METRIC = "aggregate ns/day at 2 fs (batched+learned vs plain, same GPU, fixed force error)"
 
