@@ -821,14 +821,27 @@ __global__ void __launch_bounds__(64 * MD_MASK_GROUP) k_nb_masks(const md_nb_arg
 // each replica's list segments in one numbering, and the counters the pair
 // kernel's waves take segments from, reset to 0.
 __global__ void k_nb_pack(const md_nb_args *__restrict__ T, md_bmap map, int *seg_start, int *cursor) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (blockIdx.x == 0 && threadIdx.x < 64) {
+    // the first wave: each lane one replica's segment count (two dependent
+    // loads, issued by all lanes at once rather than one replica after
+    // another), then a prefix sum across the wave, 64 replicas per pass
+    const int lane = threadIdx.x;
     int run = 0;
-    for (int r = 0; r < map.R; ++r) {
-      seg_start[r] = run;
-      run += T[r].totals[1];
+    for (int r0 = 0; r0 < map.R; r0 += 64) {
+      const int r = r0 + lane;
+      const int v = r < map.R ? T[r].totals[1] : 0;
+      int incl = v;
+      for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += u;
+      }
+      if (r < map.R) seg_start[r] = run + incl - v;
+      run += __shfl(incl, 63, 64);
     }
-    seg_start[map.R] = run;
-    for (int c = 0; c < MD_XCDS; ++c) cursor[c * MD_CURSOR_STRIDE] = 0;
+    if (lane == 0) {
+      seg_start[map.R] = run;
+      for (int c = 0; c < MD_XCDS; ++c) cursor[c * MD_CURSOR_STRIDE] = 0;
+    }
   }
   int blk;
   const md_nb_args &A = md_nb_entry(T, map, blk);
