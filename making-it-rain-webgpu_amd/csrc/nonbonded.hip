@@ -487,6 +487,15 @@ struct md_super_view {
 // centre, so |centre_J - O| < rl + rs + |hJ|.  Failing that bound, no box
 // test can pass, so skipping the image leaves the entries unchanged; the
 // bound is widened by 1e-3 nm against f32 rounding in both tests.
+// Which of two different super-clusters lists the pairs between them: the
+// lower-numbered one when the numbers' sum is even, the higher one when it is
+// odd.  Each unordered pair is listed exactly once, and every super-cluster
+// lists about half of its neighbours whatever its number (with "the lower
+// one" the first super-clusters listed all their neighbours and the last
+// almost none, and the search, one wave per super-cluster, waited on the
+// first ones).
+__device__ __forceinline__ bool md_lists_pair(int S, int SJ) { return ((S + SJ) & 1) ? S > SJ : S < SJ; }
+
 template <typename F>
 __device__ __forceinline__ void md_candidate(const md_build_ctx &c, const md_list_ptrs &L, const md_super_view &V,
                                              int J, bool enumerate, const int *K, F &&emit) {
@@ -497,7 +506,7 @@ __device__ __forceinline__ void md_candidate(const md_build_ctx &c, const md_lis
   // the image is chosen against S's origin and shared by the entry; S's own
   // clusters keep their unshifted frame (see md_for_each_image)
   int64_t dx = jx - V.ox, dy = jy - V.oy, dz = jz - V.oz;
-  bool own = J < V.first + 8;
+  bool own = J >= V.first && J < V.first + 8;
   md_for_each_image(c, dx, dy, dz, enumerate, own, K, reach + 1e-3f, [&](int64_t ex, int64_t ey, int64_t ez, int code) {
     // the lattice vector that moves J into this image
     int64_t shx = ex - dx, shy = ey - dy, shz = ez - dz;
@@ -508,8 +517,10 @@ __device__ __forceinline__ void md_candidate(const md_build_ctx &c, const md_lis
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       int I = V.first + q;
-      // J >= I only; a cluster with its own images keeps one of each +/- pair
-      if (q < V.nlocal && J >= I && !(J == I && !zero && code < 0)) {
+      // within S, J >= I only; a cluster with its own images keeps one of
+      // each +/- pair (J of another super-cluster comes only from one that
+      // S lists, md_lists_pair)
+      if (q < V.nlocal && (!own || J >= I) && !(J == I && !zero && code < 0)) {
         float ddx = md_from_fixed(jx + shx - V.cx[q]);
         float ddy = md_from_fixed(jy + shy - V.cy[q]);
         float ddz = md_from_fixed(jz + shz - V.cz[q]);
@@ -602,10 +613,10 @@ __global__ void __launch_bounds__(64) k_nb_list(const md_nb_args *__restrict__ T
     }
   }
   if (prefilter) {
-    for (int sb = S; sb < c.nsuper; sb += 64) {
+    for (int sb = 0; sb < c.nsuper; sb += 64) {
       const int SJ = sb + lane;
       bool near = false;
-      if (SJ < c.nsuper) {
+      if (SJ < c.nsuper && (SJ == S || md_lists_pair(S, SJ))) {
         const float lim = c.list_cutoff + rs + L.super_radius[SJ];
         if (SJ == S) {
           near = true;
@@ -637,13 +648,14 @@ __global__ void __launch_bounds__(64) k_nb_list(const md_nb_args *__restrict__ T
   }
   // past the candidate store, or images too far out to prefilter: scan every J
   const bool scan_all = !prefilter || ncand > MD_LIST_CANDIDATES;
-  const int chunks = scan_all ? (c.nclusters - first + 63) / 64 : (ncand + 7) / 8;
+  const int chunks = scan_all ? (c.nclusters + 63) / 64 : (ncand + 7) / 8;
   int written = 0;
   const int base_out = S * c.entries_per_super, region_end = base_out + c.entries_per_super;
   for (int chunk = 0; chunk < chunks; ++chunk) {
     int J;
     if (scan_all) {
-      J = first + 64 * chunk + lane;
+      J = 64 * chunk + lane;
+      if (J < c.nclusters && J / 8 != S && !md_lists_pair(S, J / 8)) J = c.nclusters;
     } else {
       int k = 8 * chunk + (lane >> 3);
       J = k < ncand ? cand[k] * 8 + (lane & 7) : c.nclusters;

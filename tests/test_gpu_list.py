@@ -15,11 +15,17 @@ pytestmark = pytest.mark.gpu
 
 
 def test_compressed_system_reports_list_overflow():
+    """The list's capacity is sized from the density the system is created
+    at.  Created in a box 1.5 times wider, then compressed to 0.55 of its
+    own box (about 20 times the density the list was sized for; the box
+    still fits the 1.0 nm cutoff), the search overflows and says so."""
     b = synthetic.config(1)
+    box0 = np.asarray(b.system.boxVectors, dtype=np.float64)
+    b.system.boxVectors = box0 * 1.5
     s = native.NativeSystem(b.system)
-    scale = 0.55  # about 6x the density; the box still fits the 1.0 nm cutoff
-    box = np.asarray(b.system.boxVectors, dtype=np.float64) * scale
-    s.set_box(box)
+    b.system.boxVectors = box0
+    scale = 0.55
+    s.set_box(box0 * scale)
     s.set_positions(b.state.positions * scale)
     with pytest.raises(native.MdError) as err:
         s.evaluate()
