@@ -49,12 +49,12 @@ import numpy as np  # noqa: E402
 FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
 PAIR_FLOPS = 76            # algorithmic FP32 flops per in-cutoff pair
-# HBM bytes per launch of the batched pair kernel on this ensemble with the
-# 64-entry work units the autotuner picks: FETCH_SIZE + WRITE_SIZE from two
-# rocprofv3 --pmc passes over `bench.py --traffic-probe 3 --pair-segment 64`,
-# each scaled by its reading for a 256 MiB device copy
-# (profiles/k_nb_pairs_ensemble64_traffic.md)
-PAIR_TRAFFIC_BYTES = 463.3e6
+# HBM bytes per launch of the batched pair kernel on this ensemble, by the
+# work-unit length in use (the autotuner picks 64 or 128 entries here):
+# FETCH_SIZE + WRITE_SIZE from two rocprofv3 --pmc passes over
+# `bench.py --traffic-probe 3 --pair-segment N`, each scaled by its reading
+# for a 256 MiB device copy (profiles/k_nb_pairs_ensemble64_traffic.md)
+PAIR_TRAFFIC_BYTES = {64: 465.0e6, 128: 436.7e6}
 DT_PS = 0.002
 METRIC = "aggregate ns/day at 2 fs (batched+learned vs plain, same GPU, fixed force error)"
 
@@ -280,6 +280,8 @@ def run_ensemble(args):
         log(f"autotuned pair schedule: {tune['segment_entries']} entries per unit, {tune['pair_waves']} waves "
             f"({tune['ms_per_step']:.3f} ms/step phase by phase) vs the default {tune['default_segment_entries']}, "
             f"{tune['default_pair_waves']} ({tune['default_ms_per_step']:.3f} ms/step)")
+    # the work-unit length the timed run uses (its measured traffic, if any)
+    unit_entries = args.pair_segment or (tune["segment_entries"] if tune else None)
     batch.run(max(args.warmup, 50))
     batch.run(args.steps)  # captures the graph of the timed run's length outside the timed region
     device_synchronize()
@@ -421,9 +423,9 @@ def run_ensemble(args):
             "arithmetic": "fp32 with Q32.32 int64 accumulation",
             "parallelism": "single GPU",
         },
-        "roofline": pair_roofline(pairs, phases["pairs"], PAIR_TRAFFIC_BYTES,
-                                  "profiles/k_nb_pairs_ensemble64_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
-                                  "scaled by a 256 MiB copy)"),
+        "roofline": pair_roofline(pairs, phases["pairs"], PAIR_TRAFFIC_BYTES.get(unit_entries),
+                                  f"profiles/k_nb_pairs_ensemble64_traffic.md ({unit_entries}-entry units; rocprofv3 "
+                                  "FETCH_SIZE + WRITE_SIZE, scaled by a 256 MiB copy)"),
         "phases_ms": phases,
         "integrator_roofline": {"bound": "hbm", "achieved": integ_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": integ_gbs / HBM_PEAK_GBS, "bytes_per_atom": 124},
