@@ -575,9 +575,11 @@ __global__ void k_pme_gather(const md_pme_table *__restrict__ T, md_bmap map) {
   float Fx = dfx * M[0] + dfy * M[1] + dfz * M[2];
   float Fy = dfx * M[3] + dfy * M[4] + dfz * M[5];
   float Fz = dfx * M[6] + dfy * M[7] + dfz * M[8];
-  md_atomic_add_i64(&force[i], md_to_fixed(Fx));
-  md_atomic_add_i64(&force[np + i], md_to_fixed(Fy));
-  md_atomic_add_i64(&force[2 * np + i], md_to_fixed(Fz));
+  // one thread per atom here (a slot holds one atom), and kernels on one
+  // stream run in order: a plain read-add-write, no atomics at memory
+  force[i] += md_to_fixed(Fx);
+  force[np + i] += md_to_fixed(Fy);
+  force[2 * np + i] += md_to_fixed(Fz);
 }
 
 // ---- host -------------------------------------------------------------------
